@@ -1,0 +1,187 @@
+"""Benchmark: 3D split-step stirring run at 256^3 on one MI355X.
+
+Workload (reference units, see DESIGN.md for the oscillator-unit mapping):
+256^3 grid on [-20, 20)^3, static trap V = x^2 + y^2 + z^2, beta = 4000,
+Gaussian stirring beam 10*exp(-((x-4cos t)^2 + (y-4sin t)^2)/2), dt = 2.5e-4,
+complex64 field with FP64 norm/energy and FP64 time, |psi(k)|^2 shell
+histogram every 500 steps.  A step is one Strang step over the whole grid;
+the state is a seeded synthetic Thomas-Fermi profile plus noise, resident in
+HBM before timing starts.
+
+Prints ONE JSON line.  `value` = grid-point-steps/s of the complex64 run.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N = 256
+LENGTH = 40.0
+BETA = 4000.0
+DT = 2.5e-4
+STATIC = "x**2 + y**2 + z**2"
+BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
+NK_INTERVAL = 500
+HBM_PEAK_GBS = 8000.0
+PASSES_PER_STEP = 4          # steady-state psi sweeps per real-time step (fused z pass)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def synthetic_state(x, seed=0):
+    """Thomas-Fermi profile sqrt(max(mu_TF - V, 0)/beta) + 1e-3 complex noise."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    V = X * X + Y * Y + Z * Z
+    mu_tf = (15 * BETA / (8 * np.pi)) ** 0.4        # TF chemical potential for V = r^2
+    psi = np.sqrt(np.maximum(mu_tf - V, 0) / BETA).astype(np.complex128)
+    psi += 1e-3 * (rng.standard_normal(psi.shape) + 1j * rng.standard_normal(psi.shape))
+    dx = x[1] - x[0]
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * dx ** 3)
+    return psi
+
+
+def run_engine(precision, psi0, x, k, steps, warmup):
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    solve = make_solve(N, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=precision)
+    solve.upload(psi0)
+    tm = midpoint_times(0, warmup + steps, DT)
+    solve.step(warmup, tm[:warmup])
+    solve.diagnostics(warmup * DT)
+    solve.sync()
+    done = 0
+    t0 = time.perf_counter()
+    while done < steps:
+        chunk = min(NK_INTERVAL, steps - done)
+        solve.step(chunk, tm[warmup + done: warmup + done + chunk])
+        done += chunk
+        if done % NK_INTERVAL == 0:
+            solve.diagnostics((warmup + done) * DT)      # nk histogram + FP64 scalars (syncs)
+    solve.sync()
+    elapsed = time.perf_counter() - t0
+    diag = solve.diagnostics((warmup + steps) * DT, spectrum=False)
+    return solve, elapsed, diag
+
+
+def kernel_timing(solve, steps):
+    from gpe_simulator_amd.engine import midpoint_times
+    tm = midpoint_times(0, steps, DT)
+    return solve.step_timed(steps, tm)
+
+
+def cpu_baseline(psi0, samples):
+    """The FP64 oracle (numpy.fft, single thread) on a bounded sample."""
+    from oracle import gpe_oracle as orc
+    problem = orc.Problem(N, LENGTH, "real", DT, BETA, static=STATIC, dynamic=BEAM)
+    psi = psi0.copy()
+    psi = problem.step(psi, 0.0)                     # warm-up step
+    t0 = time.perf_counter()
+    for i in range(1, samples + 1):
+        psi = problem.step(psi, i * DT)
+    elapsed = time.perf_counter() - t0
+    return N ** 3 * samples / elapsed, elapsed
+
+
+def main(argv=None):
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=2000)
+    parser.add_argument("--warmup", type=int, default=100)
+    parser.add_argument("--c128-steps", type=int, default=1000)
+    parser.add_argument("--timed-steps", type=int, default=50)
+    parser.add_argument("--cpu-samples", type=int, default=3)
+    parser.add_argument("--skip-cpu", action="store_true")
+    parser.add_argument("--skip-c128", action="store_true")
+    args = parser.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+
+    from gpe_simulator_amd import _lib
+    from oracle.gpe_oracle import get_grids
+    x, k = get_grids(N, LENGTH)
+    log("building synthetic state")
+    psi0 = synthetic_state(x)
+
+    log(f"c64 run: {args.warmup} warm-up + {args.steps} timed steps")
+    solve64, el64, diag64 = run_engine("c64", psi0, x, k, args.steps, args.warmup)
+    value = N ** 3 * args.steps / el64
+    log(f"c64: {el64 * 1e3 / args.steps:.4f} ms/step, {value:.4e} grid-pt-steps/s, mu={diag64['mu']:.9g}")
+
+    log(f"per-kernel event timing over {args.timed_steps} steps")
+    timing = kernel_timing(solve64, args.timed_steps)
+    dominant = max(timing, key=lambda kk: timing[kk][0])
+    dom_ms = timing[dominant][0] / max(1, timing[dominant][1])
+    bytes_per_launch = N ** 3 * 2 * 8                 # one read + one write sweep of complex64 psi
+    achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
+    per_kernel = {kk: {"avg_us": round(v[0] / v[1] * 1e3, 3), "launches": v[1]}
+                  for kk, v in timing.items() if v[1]}
+    solve64.close()
+
+    copy_bytes = 256 << 20
+    ms_copy = _lib.default_context().bench_copy(copy_bytes, reps=20)
+    copy_gbs = 2 * copy_bytes / (ms_copy * 1e-3) / 1e9
+
+    result_c128 = None
+    if not args.skip_c128:
+        log(f"c128 run: {args.c128_steps} steps")
+        solve128, el128, diag128 = run_engine("c128", psi0, x, k, args.c128_steps, args.warmup)
+        v128 = N ** 3 * args.c128_steps / el128
+        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / args.c128_steps,
+                       "speedup_c64_over_c128": value / v128, "mu": diag128["mu"]}
+        solve128.close()
+
+    cpu = None
+    if not args.skip_cpu:
+        log(f"CPU oracle baseline: {args.cpu_samples} steps at {N}^3")
+        rate, el = cpu_baseline(psi0, args.cpu_samples)
+        cpu = {"value": rate, "unit": "grid-pt-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_samples} Strang steps of the same 256^3 stirring workload "
+                         f"(complex128 numpy.fft, 1 thread) after 1 warm-up step, {el:.1f} s"}
+
+    step_bytes = PASSES_PER_STEP * 2 * 8
+    line = {
+        "metric": "3D SSFM grid-pt-steps/s at 256^3 c64 (% HBM roofline) vs c128; p50/p99 solve ms",
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el64 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+f64-reductions",
+        "data": "synthetic (seeded Thomas-Fermi + noise), resident in HBM",
+        "config": {"workload": "stirring run 256^3, L=40, beta=4000, dt=2.5e-4, nk every 500 steps",
+                   "grid": N, "precision": "complex64 field, FP64 reductions/time"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "step_roofline_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
+        "stream_copy_gbs": copy_gbs,
+        "per_kernel": per_kernel,
+        "mu_c64": diag64["mu"],
+        "c128": result_c128,
+        "cpu_baseline": cpu,
+        "p50_solve_ms": None,
+        "p99_solve_ms": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,315 @@
+"""ctypes binding of libgpe_hip.so (the C ABI declared in include/gpe_hip.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+libgpe_hip.so is missing or no HIP device is present, every call that needs the
+device raises GpeError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY = Path(__file__).resolve().parent / "libgpe_hip.so"
+
+GPE_OK, GPE_ERR_INVALID, GPE_ERR_NOMEM, GPE_ERR_HIP, GPE_ERR_STATE = 0, -1, -2, -3, -4
+C64, C128 = 0, 1
+REAL, IMAG = 0, 1
+TGT_XY, TGT_Z, TGT_VOL, TGT_VOL_IM = 0, 1, 2, 3
+
+PASS_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin", "z_diag", "x_diag", "z_fwd", "y_lz", "x_lz")
+
+EXPORTS = (
+    "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",
+    "gpe_solve_create", "gpe_solve_destroy", "gpe_solve_footprint", "gpe_psi_upload",
+    "gpe_psi_download", "gpe_psi_copy", "gpe_psi_gaussian", "gpe_potential_static",
+    "gpe_potential_dynamic", "gpe_volume_fill", "gpe_volume_upload", "gpe_step",
+    "gpe_norm_history", "gpe_norm", "gpe_normalise", "gpe_diagnostics", "gpe_sync",
+    "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum",
+)
+
+
+class GpeError(RuntimeError):
+    """A libgpe_hip call failed (status code and the library's message)."""
+
+    def __init__(self, code, message):
+        self.code = code
+        super().__init__(f"libgpe_hip error {code}: {message}")
+
+
+class GpeInvalid(GpeError, AssertionError):
+    """Contract violation; an AssertionError like the reference's checks."""
+
+
+class SolveDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int32), ("precision", c_int32), ("kind", c_int32), ("kcut_enable", c_int32),
+        ("dt", c_double), ("beta", c_double), ("kcut", c_double), ("kinetic_coef", c_double),
+        ("x_array", POINTER(c_double)), ("k_array", POINTER(c_double)),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(lib):
+    vp, i32, i64, dp = c_void_p, c_int32, c_int64, POINTER(c_double)
+    sig = {
+        "gpe_version": (c_char_p, []),
+        "gpe_last_error": (c_char_p, []),
+        "gpe_ctx_create": (c_int, [c_int, POINTER(vp)]),
+        "gpe_ctx_destroy": (c_int, [vp]),
+        "gpe_mem_info": (c_int, [vp, POINTER(c_size_t), POINTER(c_size_t)]),
+        "gpe_solve_create": (c_int, [vp, POINTER(SolveDesc), POINTER(vp)]),
+        "gpe_solve_destroy": (c_int, [vp]),
+        "gpe_solve_footprint": (c_int, [POINTER(SolveDesc), POINTER(c_size_t), POINTER(c_size_t)]),
+        "gpe_psi_upload": (c_int, [vp, dp, i64, i64]),
+        "gpe_psi_download": (c_int, [vp, dp, i64, i64]),
+        "gpe_psi_copy": (c_int, [vp, vp]),
+        "gpe_psi_gaussian": (c_int, [vp, c_double]),
+        "gpe_potential_static": (c_int, [vp, i32, POINTER(i32), i32, dp, i32, c_double]),
+        "gpe_potential_dynamic": (c_int, [vp, i32, POINTER(i32), i32, dp, i32]),
+        "gpe_volume_fill": (c_int, [vp, i32, c_double]),
+        "gpe_volume_upload": (c_int, [vp, i32, dp, i64, i64, i32]),
+        "gpe_step": (c_int, [vp, i64, dp]),
+        "gpe_norm_history": (c_int, [vp, dp, i64]),
+        "gpe_norm": (c_int, [vp, dp]),
+        "gpe_normalise": (c_int, [vp]),
+        "gpe_diagnostics": (c_int, [vp, c_double, dp, POINTER(i64), dp, i32]),
+        "gpe_sync": (c_int, [vp]),
+        "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
+        "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
+        "gpe_angular_momentum": (c_int, [vp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """The loaded library (raises GpeError if it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY.exists():
+                raise GpeError(GPE_ERR_STATE, f"{LIBRARY} is not built; run gpe_simulator_amd.build")
+            handle = ctypes.CDLL(str(LIBRARY))
+            _declare(handle)
+            _lib = handle
+    return _lib
+
+
+def check(code):
+    if code == GPE_OK:
+        return
+    message = lib().gpe_last_error().decode(errors="replace")
+    if code == GPE_ERR_INVALID:
+        raise GpeInvalid(code, message)
+    raise GpeError(code, message)
+
+
+def dptr(array):
+    return array.ctypes.data_as(POINTER(c_double))
+
+
+def i32ptr(array):
+    return array.ctypes.data_as(POINTER(c_int32))
+
+
+def i64ptr(array):
+    return array.ctypes.data_as(POINTER(c_int64))
+
+
+class Context:
+    """A device context (one HIP device)."""
+
+    def __init__(self, device=0):
+        handle = c_void_p()
+        check(lib().gpe_ctx_create(int(device), byref(handle)))
+        self.handle = handle
+        self.device = device
+
+    def mem_info(self):
+        free, total = c_size_t(), c_size_t()
+        check(lib().gpe_mem_info(self.handle, byref(free), byref(total)))
+        return free.value, total.value
+
+    def bench_copy(self, nbytes, reps=20):
+        out = c_double()
+        check(lib().gpe_bench_copy(self.handle, int(nbytes), int(reps), byref(out)))
+        return out.value
+
+    def close(self):
+        if self.handle:
+            lib().gpe_ctx_destroy(self.handle)
+            self.handle = None
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    with _lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+    return _default_ctx
+
+
+def make_desc(n, precision, kind, dt, beta, x_array, k_array, kcut=None, kinetic_coef=1.0):
+    x = np.ascontiguousarray(x_array, dtype=np.float64)
+    k = np.ascontiguousarray(k_array, dtype=np.float64)
+    desc = SolveDesc(n=int(n), precision=int(precision), kind=int(kind),
+                     kcut_enable=1 if kcut is not None else 0, dt=float(dt), beta=float(beta),
+                     kcut=float(kcut or 0.0), kinetic_coef=float(kinetic_coef),
+                     x_array=dptr(x), k_array=dptr(k))
+    desc._keep = (x, k)
+    return desc
+
+
+def footprint(n, precision, kind=REAL):
+    x = np.zeros(max(2, n))
+    desc = make_desc(n, precision, kind, 1.0, 0.0, x, x)
+    resident, diag = c_size_t(), c_size_t()
+    check(lib().gpe_solve_footprint(byref(desc), byref(resident), byref(diag)))
+    return resident.value, diag.value
+
+
+def nbins_for(n):
+    import math
+    return int(math.floor(math.sqrt(3) * (n // 2) + 0.5)) + 1
+
+
+class Solve:
+    """One device-resident split-step solve (psi + tables) behind the C ABI."""
+
+    def __init__(self, n, precision, kind, dt, beta, x_array, k_array, kcut=None, kinetic_coef=1.0,
+                 ctx=None):
+        self.ctx = ctx or default_context()
+        self.n, self.precision, self.kind = int(n), int(precision), int(kind)
+        self.dt, self.beta = float(dt), float(beta)
+        self.nbins = nbins_for(self.n)
+        desc = make_desc(n, precision, kind, dt, beta, x_array, k_array, kcut, kinetic_coef)
+        handle = c_void_p()
+        check(lib().gpe_solve_create(self.ctx.handle, byref(desc), byref(handle)))
+        self.handle = handle
+        self.has_dynamic = False
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gpe_solve_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def size(self):
+        return self.n ** 3
+
+    # -- field transfer
+    def upload(self, psi, offset=0):
+        data = np.ascontiguousarray(psi, dtype=np.complex128).reshape(-1)
+        check(lib().gpe_psi_upload(self.handle, dptr(data.view(np.float64)), int(offset), data.size))
+
+    def download(self, offset=0, count=None):
+        count = self.size - offset if count is None else count
+        out = np.empty(count, dtype=np.complex128)
+        check(lib().gpe_psi_download(self.handle, dptr(out.view(np.float64)), int(offset), int(count)))
+        return out
+
+    def to_numpy(self):
+        return self.download().reshape((self.n,) * 3)
+
+    def copy_from(self, other):
+        check(lib().gpe_psi_copy(self.handle, other.handle))
+
+    def gaussian(self, sigma):
+        check(lib().gpe_psi_gaussian(self.handle, float(sigma)))
+
+    # -- potentials
+    def potential_static(self, target, program, scale=1.0):
+        code, consts = program
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        consts = np.ascontiguousarray(consts, dtype=np.float64)
+        check(lib().gpe_potential_static(self.handle, int(target), i32ptr(code), code.size, dptr(consts),
+                                         consts.size, float(scale)))
+
+    def potential_dynamic(self, target, program):
+        code, consts = program
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        consts = np.ascontiguousarray(consts, dtype=np.float64)
+        check(lib().gpe_potential_dynamic(self.handle, int(target), i32ptr(code), code.size, dptr(consts),
+                                          consts.size))
+        self.has_dynamic = True
+
+    def volume_fill(self, target, value):
+        check(lib().gpe_volume_fill(self.handle, int(target), float(value)))
+
+    def volume_upload(self, target, values, offset=0, accumulate=False):
+        data = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        check(lib().gpe_volume_upload(self.handle, int(target), dptr(data), int(offset), data.size,
+                                      1 if accumulate else 0))
+
+    # -- stepping
+    def step(self, nsteps=1, t_mid=None):
+        tm = None
+        if t_mid is not None:
+            tm = np.ascontiguousarray(t_mid, dtype=np.float64)
+            if tm.size < nsteps:
+                raise ValueError("t_mid needs one time per step")
+        check(lib().gpe_step(self.handle, int(nsteps), dptr(tm) if tm is not None else None))
+
+    def step_timed(self, nsteps, t_mid=None):
+        tm = None if t_mid is None else np.ascontiguousarray(t_mid, dtype=np.float64)
+        ms = np.zeros(len(PASS_KINDS))
+        cnt = np.zeros(len(PASS_KINDS), dtype=np.int64)
+        check(lib().gpe_step_timed(self.handle, int(nsteps), dptr(tm) if tm is not None else None, dptr(ms),
+                                   i64ptr(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def norm_history(self, count):
+        out = np.empty(int(count))
+        check(lib().gpe_norm_history(self.handle, dptr(out), int(count)))
+        return out
+
+    def sync(self):
+        check(lib().gpe_sync(self.handle))
+
+    # -- reductions
+    def norm(self):
+        out = c_double()
+        check(lib().gpe_norm(self.handle, byref(out)))
+        return out.value
+
+    def angular_momentum(self):
+        out = c_double()
+        check(lib().gpe_angular_momentum(self.handle, byref(out)))
+        return out.value
+
+    def normalise(self):
+        check(lib().gpe_normalise(self.handle))
+
+    def diagnostics(self, t=0.0, spectrum=True):
+        scal = np.zeros(8)
+        counts = np.zeros(self.nbins, dtype=np.int64) if spectrum else None
+        occ = np.zeros(self.nbins) if spectrum else None
+        check(lib().gpe_diagnostics(self.handle, float(t), dptr(scal),
+                                    i64ptr(counts) if spectrum else None,
+                                    dptr(occ) if spectrum else None, self.nbins))
+        names = ("norm", "E_kin", "E_pot", "E_int", "E", "mu", "parseval")
+        result = {name: float(scal[i]) for i, name in enumerate(names)}
+        if spectrum:
+            result["counts"] = counts
+            result["occupation"] = occ
+        return result

@@ -1,0 +1,957 @@
+// C ABI of libgpe_hip.so: contexts, solves, step orchestration and
+// diagnostics.  Declarations and reference citations: include/gpe_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gpe_hip.h"
+#include "gpe_misc.h"
+
+using namespace gpe;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(GPE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)                  \
+  do {                             \
+    int r_ = (expr);               \
+    if (r_ != GPE_OK) return r_;   \
+  } while (0)
+
+constexpr int kPassKinds = 11;
+constexpr long long kDynBudget = 64LL << 20;   // bytes of time-dependent slot tables
+constexpr long long kStageElems = 1LL << 20;   // complex128 elements per staging copy
+
+struct DevProgram {
+  int* code = nullptr;
+  double* consts = nullptr;
+  int ncode = 0;
+  bool active = false;
+};
+
+struct PassArgsRaw {
+  const void* s;
+  int slot_a, slot_b;
+  int flags;
+  int hist_index;
+};
+static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<float>), "pass argument layout");
+static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<double>), "pass argument layout");
+
+int nbins_for(int n) { return int(std::floor(std::sqrt(3.0) * (n / 2) + 0.5)) + 1; }
+
+bool is_fast_side(int n) { return n == 64 || n == 96 || n == 128 || n == 192 || n == 256; }
+
+void pick_launchers(int prec, int n, PassLauncher* l, PassBlocks* b) {
+#define GPE_PICK(P, N)            \
+  if (prec == P && n == N) {      \
+    *l = launch_pass_p##P##_n##N; \
+    *b = pass_blocks_p##P##_n##N; \
+    return;                       \
+  }
+  GPE_PICK(0, 64) GPE_PICK(0, 96) GPE_PICK(0, 128) GPE_PICK(0, 192) GPE_PICK(0, 256)
+  GPE_PICK(1, 64) GPE_PICK(1, 96) GPE_PICK(1, 128) GPE_PICK(1, 192) GPE_PICK(1, 256)
+#undef GPE_PICK
+  if (prec == 0) {
+    *l = launch_pass_p0_n0;
+    *b = pass_blocks_p0_n0;
+  } else {
+    *l = launch_pass_p1_n0;
+    *b = pass_blocks_p1_n0;
+  }
+}
+
+}  // namespace
+
+struct gpe_ctx {
+  int device = 0;
+};
+
+struct gpe_solve {
+  gpe_ctx* ctx = nullptr;
+  int n = 0, prec = 0, kind = 0, masked = 0, nbins = 0;
+  double dt = 0, beta = 0, kcut = 0, coef = 1, dx3 = 0;
+  long long n3 = 0;
+  size_t esz = 0, rsz = 0;
+  hipStream_t st = nullptr;
+  PassLauncher launch = nullptr;
+  PassBlocks blocks = nullptr;
+
+  void* psi = nullptr;
+  void* work = nullptr;
+  void* tw = nullptr;
+  void* kin[3] = {nullptr, nullptr, nullptr};
+  double* xa = nullptr;
+  double* ka = nullptr;
+
+  double* m_xy = nullptr;                 // FP64 static tables
+  double* m_z = nullptr;
+  double* m_vol[2] = {nullptr, nullptr};
+  void* w_xy = nullptr;                   // working-precision static tables
+  void* w_z = nullptr;
+  void* w_vol[2] = {nullptr, nullptr};
+
+  DevProgram dyn[3];                      // XY, Z, VOL
+  int chunk = 0;                          // steps per dynamic-table chunk
+  void* d_tab[3] = {nullptr, nullptr, nullptr};   // [chunk+1][size] slots
+  void* g_tab[3] = {nullptr, nullptr, nullptr};   // diagnostic-time slot
+  double* d_tlist = nullptr;
+  double* h_tlist[2] = {nullptr, nullptr};
+  hipEvent_t tlist_ev[2] = {nullptr, nullptr};
+  int tlist_flip = 0;
+
+  double* red = nullptr;
+  long long red_cap = 0;
+  unsigned int* counter = nullptr;
+  double* scal = nullptr;
+  double* hist = nullptr;
+  long long hist_cap = 0, hist_len = 0;
+  double* nkocc = nullptr;
+  unsigned int* nkcnt = nullptr;
+  double* d_out = nullptr;
+  double* d_nkocc = nullptr;
+  long long* d_nkcnt = nullptr;
+  void* stage = nullptr;
+
+  void* desc_step = nullptr;
+  void* desc_diag = nullptr;
+  std::vector<char> h_desc_step, h_desc_diag;
+
+  bool pending = false;                   // psi_true = psi * scal[1]
+  bool timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double ms[kPassKinds] = {0};
+  long long cnt[kPassKinds] = {0};
+};
+
+namespace {
+
+int set_device(gpe_solve* s) {
+  HIPCHK(hipSetDevice(s->ctx->device));
+  return GPE_OK;
+}
+
+int dmalloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(GPE_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                   hipGetErrorString(e));
+  }
+  return GPE_OK;
+}
+
+template <typename T>
+int dzalloc(T** p, size_t bytes) {
+  CHK(dmalloc((void**)p, bytes));
+  HIPCHK(hipMemset(*p, 0, bytes));
+  return GPE_OK;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+template <typename R>
+void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
+  std::memset(&d, 0, sizeof(d));
+  const long long nn = (long long)s->n * s->n;
+  d.psi = (cpx<R>*)s->psi;
+  d.work = (cpx<R>*)s->work;
+  d.tw = (const cpx<R>*)s->tw;
+  d.kin_x = (const cpx<R>*)s->kin[0];
+  d.kin_y = (const cpx<R>*)s->kin[1];
+  d.kin_z = (const cpx<R>*)s->kin[2];
+  d.karr = s->ka;
+  d.xarr = s->xa;
+  if (s->dyn[0].active) {
+    d.pxy = (const R*)(diag ? s->g_tab[0] : s->d_tab[0]);
+    d.pxy_ss = diag ? 0 : nn;
+  } else {
+    d.pxy = (const R*)s->w_xy;
+    d.pxy_ss = 0;
+  }
+  if (s->dyn[1].active) {
+    d.pz = (const R*)(diag ? s->g_tab[1] : s->d_tab[1]);
+    d.pz_ss = diag ? 0 : s->n;
+  } else {
+    d.pz = (const R*)s->w_z;
+    d.pz_ss = 0;
+  }
+  d.vre = (const R*)s->w_vol[0];
+  d.vim = (const R*)s->w_vol[1];
+  if (s->dyn[2].active) {
+    d.vdyn = (const R*)(diag ? s->g_tab[2] : s->d_tab[2]);
+    d.vdyn_ss = diag ? 0 : s->n3;
+  }
+  d.red = s->red;
+  d.counter = s->counter;
+  d.scal = s->scal;
+  d.hist = s->hist;
+  d.nkocc = s->nkocc;
+  d.nkcnt = s->nkcnt;
+  d.dx3 = s->dx3;
+  d.kcut2 = s->kcut * s->kcut;
+  d.beta = R(s->beta);
+  d.hdt = R(s->dt / 2);
+  d.n = s->n;
+  d.nbins = s->nbins;
+  d.imag = s->kind == GPE_IMAG ? 1 : 0;
+  d.masked = s->masked;
+}
+
+template <typename R>
+int refresh_desc_t(gpe_solve* s) {
+  s->h_desc_step.resize(sizeof(SolveDev<R>));
+  s->h_desc_diag.resize(sizeof(SolveDev<R>));
+  fill_desc<R>(s, *reinterpret_cast<SolveDev<R>*>(s->h_desc_step.data()), false);
+  fill_desc<R>(s, *reinterpret_cast<SolveDev<R>*>(s->h_desc_diag.data()), true);
+  HIPCHK(hipMemcpyAsync(s->desc_step, s->h_desc_step.data(), sizeof(SolveDev<R>), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(s->desc_diag, s->h_desc_diag.data(), sizeof(SolveDev<R>), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int refresh_desc(gpe_solve* s) { return s->prec == GPE_C64 ? refresh_desc_t<float>(s) : refresh_desc_t<double>(s); }
+
+int ensure_red(gpe_solve* s, long long need) {
+  if (need <= s->red_cap) return GPE_OK;
+  HIPCHK(hipStreamSynchronize(s->st));
+  dfree(s->red);
+  s->red = nullptr;
+  s->red_cap = 0;
+  CHK(dzalloc(&s->red, need * sizeof(double)));
+  s->red_cap = need;
+  return refresh_desc(s);
+}
+
+int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
+  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index};
+  if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->st));
+  hipError_t e = s->launch(kind, &a, s->n, 1, s->st);
+  if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
+  if (s->timing) {
+    HIPCHK(hipEventRecord(s->ev[1], s->st));
+    HIPCHK(hipEventSynchronize(s->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    s->ms[kind] += ms;
+    s->cnt[kind] += 1;
+  }
+  return GPE_OK;
+}
+
+template <typename R>
+int materialize_t(gpe_solve* s) {
+  if (!s->pending) return GPE_OK;
+  launch_scale<R>(s->psi, s->n3, s->scal, s->st);
+  HIPCHK(hipGetLastError());
+  s->pending = false;
+  return GPE_OK;
+}
+int materialize(gpe_solve* s) { return s->prec == GPE_C64 ? materialize_t<float>(s) : materialize_t<double>(s); }
+
+long long target_size(gpe_solve* s, int target) {
+  if (target == GPE_TGT_XY) return (long long)s->n * s->n;
+  if (target == GPE_TGT_Z) return s->n;
+  return s->n3;
+}
+
+// Checks a postfix program: opcodes, constant indices, stack discipline and
+// the variables allowed for the target.
+int validate_program(const int32_t* code, int32_t ncode, int32_t nconst, int target, bool allow_t) {
+  if (!code || ncode <= 0 || ncode > kVmMaxCode) return fail(GPE_ERR_INVALID, "program: bad code length");
+  int sp = 0;
+  for (int pc = 0; pc < ncode; ++pc) {
+    const int op = code[pc];
+    switch (op) {
+      case OP_CONST:
+        if (pc + 1 >= ncode || code[pc + 1] < 0 || code[pc + 1] >= nconst)
+          return fail(GPE_ERR_INVALID, "program: constant index out of range");
+        ++pc;
+        ++sp;
+        break;
+      case OP_X:
+      case OP_Y:
+        if (target == GPE_TGT_Z) return fail(GPE_ERR_INVALID, "program: z-table program uses x or y");
+        ++sp;
+        break;
+      case OP_Z:
+        if (target == GPE_TGT_XY) return fail(GPE_ERR_INVALID, "program: xy-table program uses z");
+        ++sp;
+        break;
+      case OP_T:
+        if (!allow_t) return fail(GPE_ERR_INVALID, "program: static program uses t");
+        ++sp;
+        break;
+      case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_POW: case OP_MAX:
+        if (sp < 2) return fail(GPE_ERR_INVALID, "program: stack underflow");
+        --sp;
+        break;
+      case OP_NEG: case OP_SIN: case OP_COS: case OP_TAN: case OP_EXP: case OP_SQRT: case OP_TANH:
+      case OP_FABS: case OP_LOG:
+        if (sp < 1) return fail(GPE_ERR_INVALID, "program: stack underflow");
+        break;
+      default:
+        return fail(GPE_ERR_INVALID, "program: unknown opcode " + std::to_string(op));
+    }
+    if (sp > kVmStack) return fail(GPE_ERR_INVALID, "program: expression too deep");
+  }
+  if (sp != 1) return fail(GPE_ERR_INVALID, "program: must leave exactly one value");
+  return GPE_OK;
+}
+
+int upload_program(DevProgram& p, const int32_t* code, int32_t ncode, const double* consts, int32_t nconst) {
+  dfree(p.code);
+  dfree(p.consts);
+  p.code = nullptr;
+  p.consts = nullptr;
+  CHK(dmalloc((void**)&p.code, sizeof(int) * ncode));
+  CHK(dmalloc((void**)&p.consts, sizeof(double) * std::max(1, nconst)));
+  HIPCHK(hipMemcpy(p.code, code, sizeof(int) * ncode, hipMemcpyHostToDevice));
+  if (nconst > 0) HIPCHK(hipMemcpy(p.consts, consts, sizeof(double) * nconst, hipMemcpyHostToDevice));
+  p.ncode = ncode;
+  p.active = true;
+  return GPE_OK;
+}
+
+int ensure_volume(gpe_solve* s, int comp) {
+  if (s->m_vol[comp]) return GPE_OK;
+  CHK(dzalloc(&s->m_vol[comp], s->n3 * sizeof(double)));
+  CHK(dzalloc(&s->w_vol[comp], s->n3 * s->rsz));
+  return refresh_desc(s);
+}
+
+int recast(gpe_solve* s, const double* master, void* out, long long size) {
+  if (s->prec == GPE_C64)
+    launch_cast<float>(master, out, size, s->st);
+  else
+    launch_cast<double>(master, out, size, s->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+// (Re)allocate the time-dependent slot tables for the active programs.
+int layout_dynamic(gpe_solve* s) {
+  HIPCHK(hipStreamSynchronize(s->st));
+  long long biggest = 0;
+  for (int g = 0; g < 3; ++g)
+    if (s->dyn[g].active) biggest = std::max(biggest, target_size(s, g == 2 ? GPE_TGT_VOL : g) * (long long)s->rsz);
+  int chunk = (int)std::max(1LL, std::min(256LL, kDynBudget / std::max(1LL, biggest) - 1));
+  s->chunk = chunk;
+  for (int g = 0; g < 3; ++g) {
+    dfree(s->d_tab[g]);
+    dfree(s->g_tab[g]);
+    s->d_tab[g] = s->g_tab[g] = nullptr;
+    if (!s->dyn[g].active) continue;
+    const long long sz = target_size(s, g == 2 ? GPE_TGT_VOL : g);
+    CHK(dzalloc(&s->d_tab[g], (size_t)(chunk + 1) * sz * s->rsz));
+    CHK(dzalloc(&s->g_tab[g], (size_t)sz * s->rsz));
+  }
+  dfree(s->d_tlist);
+  s->d_tlist = nullptr;
+  CHK(dzalloc(&s->d_tlist, (size_t)(chunk + 2) * sizeof(double)));
+  for (int f = 0; f < 2; ++f) {
+    if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
+    s->h_tlist[f] = nullptr;
+    HIPCHK(hipHostMalloc((void**)&s->h_tlist[f], (size_t)(chunk + 2) * sizeof(double), hipHostMallocDefault));
+    if (!s->tlist_ev[f]) HIPCHK(hipEventCreateWithFlags(&s->tlist_ev[f], hipEventDisableTiming));
+  }
+  return refresh_desc(s);
+}
+
+bool any_dynamic(gpe_solve* s) { return s->dyn[0].active || s->dyn[1].active || s->dyn[2].active; }
+
+// Evaluate every active time-dependent program at times t[0..nslots) into
+// slot tables (diag == false) or the single diagnostic slot (diag == true).
+int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag) {
+  const int f = s->tlist_flip;
+  s->tlist_flip ^= 1;
+  HIPCHK(hipEventSynchronize(s->tlist_ev[f]));
+  std::memcpy(s->h_tlist[f], t, sizeof(double) * nslots);
+  double* dst_t = s->d_tlist;
+  HIPCHK(hipMemcpyAsync(dst_t, s->h_tlist[f], sizeof(double) * nslots, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipEventRecord(s->tlist_ev[f], s->st));
+  for (int g = 0; g < 3; ++g) {
+    if (!s->dyn[g].active) continue;
+    const int target = g == 2 ? GPE_TGT_VOL : g;
+    const long long size = target_size(s, target);
+    const double* master = g == 0 ? s->m_xy : (g == 1 ? s->m_z : nullptr);
+    void* out = diag ? s->g_tab[g] : s->d_tab[g];
+    if (s->prec == GPE_C64)
+      launch_vm_dynamic<float>(s->dyn[g].code, s->dyn[g].ncode, s->dyn[g].consts, target, s->n, s->xa, size,
+                               master, dst_t, nslots, out, s->st);
+    else
+      launch_vm_dynamic<double>(s->dyn[g].code, s->dyn[g].ncode, s->dyn[g].consts, target, s->n, s->xa, size,
+                                master, dst_t, nslots, out, s->st);
+    HIPCHK(hipGetLastError());
+  }
+  return GPE_OK;
+}
+
+int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
+  const bool dyn = any_dynamic(s);
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t cs = dyn ? std::min<int64_t>(s->chunk, nsteps - done) : nsteps;
+    if (dyn) {
+      const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
+      CHK(eval_dynamic(s, tmid + done, nslots, false));
+    }
+    for (int64_t q = 0; q < cs; ++q) {
+      const int64_t step = done + q;
+      const int sa = dyn ? (int)q : 0, sb = dyn ? (int)q + 1 : 0;
+      if (step == 0) {
+        CHK(pass(s, K_Z_FIRST, false, sa, 0, s->pending ? F_PRESCALE : 0));
+        s->pending = false;
+      }
+      CHK(pass(s, K_Y_FWD, false, 0, 0, 0));
+      CHK(pass(s, K_X_KIN, false, 0, 0, 0));
+      CHK(pass(s, K_Y_INV, false, 0, 0, 0));
+      if (step == nsteps - 1)
+        CHK(pass(s, K_Z_LAST, false, sa, 0, 0));
+      else
+        CHK(pass(s, K_Z_MID, false, sa, sb, 0));
+    }
+    done += cs;
+  }
+  return GPE_OK;
+}
+
+int step_imag(gpe_solve* s, int64_t nsteps) {
+  if (nsteps > s->hist_cap) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->hist);
+    s->hist = nullptr;
+    s->hist_cap = 0;
+    const long long cap = std::max<long long>(nsteps, 1024);
+    CHK(dzalloc(&s->hist, cap * sizeof(double)));
+    s->hist_cap = cap;
+    CHK(refresh_desc(s));
+  }
+  CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
+  for (int64_t step = 0; step < nsteps; ++step) {
+    CHK(pass(s, K_Z_FIRST, false, 0, 0, s->pending ? F_PRESCALE : 0));
+    CHK(pass(s, K_Y_FWD, false, 0, 0, 0));
+    CHK(pass(s, K_X_KIN, false, 0, 0, 0));
+    CHK(pass(s, K_Y_INV, false, 0, 0, 0));
+    CHK(pass(s, K_Z_LAST, false, 0, 0, F_NORM, (int)step));
+    s->pending = true;
+  }
+  s->hist_len = nsteps;
+  return GPE_OK;
+}
+
+int do_step(gpe_solve* s, int64_t nsteps, const double* tmid) {
+  if (nsteps < 0) return fail(GPE_ERR_INVALID, "nsteps must be >= 0");
+  if (nsteps == 0) return GPE_OK;
+  if (any_dynamic(s) && !tmid)
+    return fail(GPE_ERR_INVALID, "t_mid is required: a time-dependent potential is registered");
+  CHK(set_device(s));
+  if (s->kind == GPE_REAL) return step_real(s, nsteps, tmid);
+  return step_imag(s, nsteps);
+}
+
+template <typename R>
+int norm_t(gpe_solve* s) {
+  CHK(materialize_t<R>(s));
+  CHK(ensure_red(s, kNormBlocks));
+  launch_norm<R>(s->psi, s->n3, s->red, s->counter, s->scal, s->dx3, 0, s->st);
+  HIPCHK(hipGetLastError());
+  return GPE_OK;
+}
+int norm_any(gpe_solve* s) { return s->prec == GPE_C64 ? norm_t<float>(s) : norm_t<double>(s); }
+
+int ensure_diag(gpe_solve* s) {
+  if (s->work) return GPE_OK;
+  CHK(dzalloc(&s->work, s->n3 * s->esz));
+  const long long bx = s->blocks(K_X_DIAG, s->n);
+  CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double)));
+  CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int)));
+  CHK(dzalloc(&s->d_out, 16 * sizeof(double)));
+  CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double)));
+  CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long)));
+  const long long need = std::max(s->blocks(K_Z_DIAG, s->n) * 3, bx * 2);
+  if (need > s->red_cap) {
+    dfree(s->red);
+    s->red = nullptr;
+    CHK(dzalloc(&s->red, need * sizeof(double)));
+    s->red_cap = need;
+  }
+  return refresh_desc(s);
+}
+
+int ensure_stage(gpe_solve* s) {
+  if (s->stage) return GPE_OK;
+  return dmalloc(&s->stage, kStageElems * 16);
+}
+
+int check_range(gpe_solve* s, int64_t offset, int64_t count) {
+  if (offset < 0 || count < 0 || offset + count > s->n3) return fail(GPE_ERR_INVALID, "element range out of bounds");
+  return GPE_OK;
+}
+
+}  // namespace
+
+// ===================================================================== ABI ==
+extern "C" {
+
+const char* gpe_version(void) { return "gpe_hip 0.1 (gfx950)"; }
+
+const char* gpe_last_error(void) { return g_err.c_str(); }
+
+int gpe_ctx_create(int device, gpe_ctx** out) {
+  if (!out) return fail(GPE_ERR_INVALID, "out is null");
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(GPE_ERR_INVALID, "no HIP device " + std::to_string(device));
+  HIPCHK(hipSetDevice(device));
+  gpe_ctx* c = new gpe_ctx();
+  c->device = device;
+  *out = c;
+  return GPE_OK;
+}
+
+int gpe_ctx_destroy(gpe_ctx* ctx) {
+  delete ctx;
+  return GPE_OK;
+}
+
+int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+  if (!ctx || !free_bytes || !total_bytes) return fail(GPE_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return GPE_OK;
+}
+
+int gpe_solve_footprint(const gpe_solve_desc* d, size_t* resident, size_t* diag) {
+  if (!d || !resident || !diag) return fail(GPE_ERR_INVALID, "null argument");
+  const size_t n = (size_t)d->n, n3 = n * n * n;
+  const size_t esz = d->precision == GPE_C64 ? 8 : 16, rsz = esz / 2;
+  *resident = n3 * esz + n * n * (8 + rsz) + 16 * n * 8 + (4 << 20);
+  const int nb = nbins_for(d->n);
+  *diag = n3 * esz + (n * n) * (size_t)nb * 12 + kStageElems * 16;
+  return GPE_OK;
+}
+
+int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
+  if (!ctx || !d || !out) return fail(GPE_ERR_INVALID, "null argument");
+  if (d->n < 2 || d->n > 1024) return fail(GPE_ERR_INVALID, "n must be in 2..1024");
+  if (d->precision != GPE_C64 && d->precision != GPE_C128) return fail(GPE_ERR_INVALID, "bad precision");
+  if (d->kind != GPE_REAL && d->kind != GPE_IMAG) return fail(GPE_ERR_INVALID, "bad kind");
+  if (!(d->dt > 0) || !std::isfinite(d->dt)) return fail(GPE_ERR_INVALID, "dt must be finite and positive");
+  if (!std::isfinite(d->beta)) return fail(GPE_ERR_INVALID, "beta must be finite");
+  if (!d->x_array || !d->k_array) return fail(GPE_ERR_INVALID, "x_array and k_array are required");
+  if (d->kcut_enable && d->kind == GPE_IMAG)
+    return fail(GPE_ERR_INVALID, "the momentum cutoff is available only for real time evolution");
+  HIPCHK(hipSetDevice(ctx->device));
+
+  gpe_solve* s = new gpe_solve();
+  s->ctx = ctx;
+  s->n = d->n;
+  s->prec = d->precision;
+  s->kind = d->kind;
+  s->masked = d->kcut_enable ? 1 : 0;
+  s->dt = d->dt;
+  s->beta = d->beta;
+  s->kcut = d->kcut;
+  s->coef = d->kinetic_coef == 0.0 ? 1.0 : d->kinetic_coef;
+  s->n3 = (long long)d->n * d->n * d->n;
+  s->esz = s->prec == GPE_C64 ? 8 : 16;
+  s->rsz = s->esz / 2;
+  s->nbins = nbins_for(d->n);
+  const double dx = d->x_array[1] - d->x_array[0];
+  s->dx3 = dx * dx * dx;
+  pick_launchers(s->prec, s->n, &s->launch, &s->blocks);
+
+  auto bail = [&](int code) {
+    gpe_solve_destroy(s);
+    return code;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(GPE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  int r;
+  const int n = s->n;
+  if ((r = dzalloc(&s->psi, s->n3 * s->esz)) || (r = dzalloc(&s->tw, n * s->esz)) ||
+      (r = dzalloc(&s->kin[0], n * s->esz)) || (r = dzalloc(&s->kin[1], n * s->esz)) ||
+      (r = dzalloc(&s->kin[2], n * s->esz)) || (r = dzalloc(&s->xa, n * sizeof(double))) ||
+      (r = dzalloc(&s->ka, n * sizeof(double))) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double))) ||
+      (r = dzalloc(&s->m_z, n * sizeof(double))) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz)) ||
+      (r = dzalloc(&s->w_z, n * s->rsz)) || (r = dzalloc(&s->counter, 64)) ||
+      (r = dzalloc(&s->scal, 8 * sizeof(double))) ||
+      (r = dmalloc(&s->desc_step, sizeof(SolveDev<double>))) ||
+      (r = dmalloc(&s->desc_diag, sizeof(SolveDev<double>))))
+    return bail(r);
+  if (hipMemcpy(s->xa, d->x_array, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->ka, d->k_array, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
+    return bail(fail(GPE_ERR_HIP, "setup copy failed"));
+  const double inv_n3 = 1.0 / double(s->n3);
+  const int imag = s->kind == GPE_IMAG ? 1 : 0;
+  if (s->prec == GPE_C64) {
+    launch_twiddle<float>(s->tw, n, s->st);
+    launch_kinetic<float>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
+  } else {
+    launch_twiddle<double>(s->tw, n, s->st);
+    launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
+  }
+  if (hipGetLastError() != hipSuccess) return bail(fail(GPE_ERR_HIP, "table kernels failed to launch"));
+  if ((r = ensure_red(s, std::max<long long>(kNormBlocks, s->blocks(K_Z_LAST, n))))) return bail(r);
+  if ((r = refresh_desc(s))) return bail(r);
+  *out = s;
+  return GPE_OK;
+}
+
+int gpe_solve_destroy(gpe_solve* s) {
+  if (!s) return GPE_OK;
+  (void)hipSetDevice(s->ctx->device);
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  void* ptrs[] = {s->psi, s->work, s->tw, s->kin[0], s->kin[1], s->kin[2], s->xa, s->ka, s->m_xy, s->m_z,
+                  s->m_vol[0], s->m_vol[1], s->w_xy, s->w_z, s->w_vol[0], s->w_vol[1], s->d_tab[0], s->d_tab[1],
+                  s->d_tab[2], s->g_tab[0], s->g_tab[1], s->g_tab[2], s->d_tlist, s->red, s->counter, s->scal,
+                  s->hist, s->nkocc, s->nkcnt, s->d_out, s->d_nkocc, s->d_nkcnt, s->stage, s->desc_step,
+                  s->desc_diag};
+  for (void* p : ptrs) dfree(p);
+  for (auto& p : s->dyn) {
+    dfree(p.code);
+    dfree(p.consts);
+  }
+  for (int f = 0; f < 2; ++f) {
+    if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
+    if (s->tlist_ev[f]) (void)hipEventDestroy(s->tlist_ev[f]);
+  }
+  for (auto ev : s->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+  return GPE_OK;
+}
+
+int gpe_psi_upload(gpe_solve* s, const double* host, int64_t offset, int64_t count) {
+  if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(check_range(s, offset, count));
+  CHK(set_device(s));
+  CHK(materialize(s));
+  if (s->prec == GPE_C128) {
+    HIPCHK(hipMemcpyAsync((char*)s->psi + offset * 16, host, count * 16, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return GPE_OK;
+  }
+  CHK(ensure_stage(s));
+  for (int64_t done = 0; done < count; done += kStageElems) {
+    const int64_t c = std::min<int64_t>(kStageElems, count - done);
+    HIPCHK(hipMemcpyAsync(s->stage, host + 2 * done, c * 16, hipMemcpyHostToDevice, s->st));
+    launch_from_c128<float>(s->stage, (char*)s->psi + (offset + done) * 8, c, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return GPE_OK;
+}
+
+int gpe_psi_download(gpe_solve* s, double* host, int64_t offset, int64_t count) {
+  if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(check_range(s, offset, count));
+  CHK(set_device(s));
+  CHK(materialize(s));
+  if (s->prec == GPE_C128) {
+    HIPCHK(hipMemcpyAsync(host, (char*)s->psi + offset * 16, count * 16, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return GPE_OK;
+  }
+  CHK(ensure_stage(s));
+  for (int64_t done = 0; done < count; done += kStageElems) {
+    const int64_t c = std::min<int64_t>(kStageElems, count - done);
+    launch_to_c128<float>((char*)s->psi + (offset + done) * 8, s->stage, c, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host + 2 * done, s->stage, c * 16, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return GPE_OK;
+}
+
+int gpe_psi_copy(gpe_solve* dst, gpe_solve* src) {
+  if (!dst || !src) return fail(GPE_ERR_INVALID, "null argument");
+  if (dst->n != src->n) return fail(GPE_ERR_INVALID, "psi copy between different grids");
+  CHK(set_device(src));
+  CHK(materialize(src));
+  HIPCHK(hipStreamSynchronize(src->st));
+  CHK(set_device(dst));
+  dst->pending = false;
+  if (dst->prec == src->prec) {
+    HIPCHK(hipMemcpyAsync(dst->psi, src->psi, src->n3 * src->esz, hipMemcpyDeviceToDevice, dst->st));
+  } else if (src->prec == GPE_C64) {
+    launch_to_c128<float>(src->psi, dst->psi, src->n3, dst->st);
+  } else {
+    launch_from_c128<float>(src->psi, dst->psi, src->n3, dst->st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(dst->st));
+  return GPE_OK;
+}
+
+int gpe_psi_gaussian(gpe_solve* s, double sigma) {
+  if (!s || !(sigma > 0)) return fail(GPE_ERR_INVALID, "sigma must be positive");
+  CHK(set_device(s));
+  s->pending = false;
+  if (s->prec == GPE_C64)
+    launch_gaussian<float>(s->psi, s->xa, s->n, sigma, s->st);
+  else
+    launch_gaussian<double>(s->psi, s->xa, s->n, sigma, s->st);
+  HIPCHK(hipGetLastError());
+  CHK(norm_any(s));
+  s->pending = true;
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_potential_static(gpe_solve* s, int32_t target, const int32_t* code, int32_t ncode, const double* consts,
+                         int32_t nconst, double scale) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (target < GPE_TGT_XY || target > GPE_TGT_VOL_IM) return fail(GPE_ERR_INVALID, "bad target");
+  if (target == GPE_TGT_VOL_IM && s->kind == GPE_IMAG)
+    return fail(GPE_ERR_INVALID, "imaginary time evolution requires a real static potential");
+  CHK(validate_program(code, ncode, nconst, target, false));
+  CHK(set_device(s));
+  DevProgram p;
+  CHK(upload_program(p, code, ncode, consts, nconst));
+  const long long size = target_size(s, target);
+  double* master;
+  void* work;
+  if (target == GPE_TGT_XY) {
+    master = s->m_xy;
+    work = s->w_xy;
+  } else if (target == GPE_TGT_Z) {
+    master = s->m_z;
+    work = s->w_z;
+  } else {
+    const int comp = target == GPE_TGT_VOL ? 0 : 1;
+    CHK(ensure_volume(s, comp));
+    master = s->m_vol[comp];
+    work = s->w_vol[comp];
+  }
+  launch_vm_static(p.code, p.ncode, p.consts, target, s->n, s->xa, size, scale, master, s->st);
+  HIPCHK(hipGetLastError());
+  int r = recast(s, master, work, size);
+  dfree(p.code);
+  dfree(p.consts);
+  return r;
+}
+
+int gpe_potential_dynamic(gpe_solve* s, int32_t target, const int32_t* code, int32_t ncode, const double* consts,
+                          int32_t nconst) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (target < GPE_TGT_XY || target > GPE_TGT_VOL) return fail(GPE_ERR_INVALID, "bad dynamic target");
+  if (s->kind == GPE_IMAG)
+    return fail(GPE_ERR_INVALID, "Imaginary time evolution does not accept a time dependent potential.");
+  CHK(validate_program(code, ncode, nconst, target, true));
+  CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  CHK(upload_program(s->dyn[target], code, ncode, consts, nconst));
+  return layout_dynamic(s);
+}
+
+int gpe_volume_fill(gpe_solve* s, int32_t target, double value) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (target != GPE_TGT_VOL && target != GPE_TGT_VOL_IM) return fail(GPE_ERR_INVALID, "bad volume target");
+  if (target == GPE_TGT_VOL_IM && s->kind == GPE_IMAG)
+    return fail(GPE_ERR_INVALID, "imaginary time evolution requires a real static potential");
+  CHK(set_device(s));
+  const int comp = target == GPE_TGT_VOL ? 0 : 1;
+  CHK(ensure_volume(s, comp));
+  launch_fill_f64(s->m_vol[comp], s->n3, value, s->st);
+  HIPCHK(hipGetLastError());
+  return recast(s, s->m_vol[comp], s->w_vol[comp], s->n3);
+}
+
+int gpe_volume_upload(gpe_solve* s, int32_t target, const double* host, int64_t offset, int64_t count,
+                      int32_t accumulate) {
+  if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  if (target != GPE_TGT_VOL && target != GPE_TGT_VOL_IM) return fail(GPE_ERR_INVALID, "bad volume target");
+  if (target == GPE_TGT_VOL_IM && s->kind == GPE_IMAG)
+    return fail(GPE_ERR_INVALID, "imaginary time evolution requires a real static potential");
+  CHK(check_range(s, offset, count));
+  CHK(set_device(s));
+  const int comp = target == GPE_TGT_VOL ? 0 : 1;
+  CHK(ensure_volume(s, comp));
+  if (!accumulate) {
+    HIPCHK(hipMemcpy(s->m_vol[comp] + offset, host, count * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    CHK(ensure_stage(s));
+    const int64_t per = kStageElems * 2;   // doubles per staging copy
+    for (int64_t done = 0; done < count; done += per) {
+      const int64_t c = std::min<int64_t>(per, count - done);
+      HIPCHK(hipMemcpyAsync(s->stage, host + done, c * sizeof(double), hipMemcpyHostToDevice, s->st));
+      launch_add_f64(s->m_vol[comp] + offset + done, (const double*)s->stage, c, s->st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s->st));
+    }
+  }
+  return recast(s, s->m_vol[comp], s->w_vol[comp], s->n3);
+}
+
+int gpe_step(gpe_solve* s, int64_t nsteps, const double* t_mid) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  return do_step(s, nsteps, t_mid);
+}
+
+int gpe_step_timed(gpe_solve* s, int64_t nsteps, const double* t_mid, double* ms_per_kind,
+                   int64_t* launches_per_kind) {
+  if (!s || !ms_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
+  for (int k = 0; k < kPassKinds; ++k) {
+    s->ms[k] = 0;
+    s->cnt[k] = 0;
+  }
+  s->timing = true;
+  int r = do_step(s, nsteps, t_mid);
+  s->timing = false;
+  for (int k = 0; k < kPassKinds; ++k) {
+    ms_per_kind[k] = s->ms[k];
+    launches_per_kind[k] = s->cnt[k];
+  }
+  return r;
+}
+
+int gpe_norm_history(gpe_solve* s, double* out, int64_t count) {
+  if (!s || (!out && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  if (count > s->hist_len) return fail(GPE_ERR_STATE, "fewer steps recorded than requested");
+  CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpy(out, s->hist + (s->hist_len - count), count * sizeof(double), hipMemcpyDeviceToHost));
+  return GPE_OK;
+}
+
+int gpe_norm(gpe_solve* s, double* out) {
+  if (!s || !out) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(set_device(s));
+  CHK(norm_any(s));
+  HIPCHK(hipMemcpyAsync(out, s->scal, sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_normalise(gpe_solve* s) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  CHK(set_device(s));
+  CHK(norm_any(s));
+  s->pending = true;
+  return GPE_OK;
+}
+
+int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, double* occupation, int32_t nbins) {
+  if (!s || !scalars) return fail(GPE_ERR_INVALID, "null argument");
+  if ((counts || occupation) && nbins != s->nbins)
+    return fail(GPE_ERR_INVALID, "nbins must be " + std::to_string(s->nbins));
+  CHK(set_device(s));
+  CHK(ensure_diag(s));
+  if (any_dynamic(s)) CHK(eval_dynamic(s, &t, 1, true));
+  const int pre = s->pending ? F_PRESCALE : 0;
+  const long long bz = s->blocks(K_Z_DIAG, s->n), bx = s->blocks(K_X_DIAG, s->n);
+  CHK(pass(s, K_Z_DIAG, true, 0, 0, pre | F_OUT_WORK));
+  launch_finish_sums(s->red, bz, 3, s->d_out, s->st);
+  HIPCHK(hipGetLastError());
+  CHK(pass(s, K_Y_FWD, true, 0, 0, F_IN_WORK | F_OUT_WORK));
+  CHK(pass(s, K_X_DIAG, true, 0, 0, F_IN_WORK));
+  launch_finish_sums(s->red, bx, 2, s->d_out + 3, s->st);
+  launch_finish_nk(s->nkocc, s->nkcnt, bx, s->nbins, s->d_nkocc, s->d_nkcnt, s->st);
+  HIPCHK(hipGetLastError());
+  double h[5];
+  HIPCHK(hipMemcpyAsync(h, s->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (occupation)
+    HIPCHK(hipMemcpyAsync(occupation, s->d_nkocc, s->nbins * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  if (counts) HIPCHK(hipMemcpyAsync(counts, s->d_nkcnt, s->nbins * sizeof(long long), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  const double n3 = double(s->n3);
+  const double norm = s->dx3 * h[0];
+  const double ekin = h[3] * s->dx3 / n3;
+  const double epot = s->dx3 * h[1];
+  const double eint = 0.5 * s->beta * s->dx3 * h[2];
+  scalars[0] = norm;
+  scalars[1] = ekin;
+  scalars[2] = epot;
+  scalars[3] = eint;
+  scalars[4] = ekin + epot + eint;
+  scalars[5] = (ekin + epot + 2.0 * eint) / norm;
+  scalars[6] = h[4] * s->dx3 / n3;
+  return GPE_OK;
+}
+
+int gpe_angular_momentum(gpe_solve* s, double* out) {
+  if (!s || !out) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(set_device(s));
+  const long long bx = s->blocks(K_X_LZ, s->n);
+  CHK(ensure_red(s, bx));
+  if (!s->d_out) CHK(dzalloc(&s->d_out, 16 * sizeof(double)));
+  const int pre = s->pending ? F_PRESCALE : 0;
+  CHK(pass(s, K_Y_LZ, true, 0, 0, pre));
+  launch_finish_sums(s->red, bx, 1, s->d_out + 8, s->st);
+  CHK(pass(s, K_X_LZ, true, 0, 0, pre));
+  launch_finish_sums(s->red, bx, 1, s->d_out + 9, s->st);
+  HIPCHK(hipGetLastError());
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, s->d_out + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  *out = (h[0] - h[1]) * s->dx3 / double(s->n);
+  return GPE_OK;
+}
+
+int gpe_sync(gpe_solve* s) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_bench_copy(gpe_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy) {
+  if (!ctx || !ms_per_copy || bytes <= 0 || bytes % 16 || reps <= 0) return fail(GPE_ERR_INVALID, "bad argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  void *a = nullptr, *b = nullptr;
+  CHK(dmalloc(&a, bytes));
+  if (int r = dmalloc(&b, bytes)) {
+    dfree(a);
+    return r;
+  }
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipMemsetAsync(a, 0, bytes, st));
+  launch_copy(a, b, bytes, st);
+  HIPCHK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) launch_copy(a, b, bytes, st);
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  dfree(a);
+  dfree(b);
+  return GPE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+// Device-side building blocks shared by every kernel of libgpe_hip.so:
+// complex arithmetic, compile-time trigonometry, static loops and the
+// in-register DFT codelets used by the pencil FFT passes.
+//
+// Written for gfx950 (CDNA4): 64-wide wavefronts, FP64 at full vector rate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace gpe {
+
+// ---------------------------------------------------------------- complex --
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) {
+  return cpx<R>{a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) {
+  return cpx<R>{a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return cpx<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) {
+  return cpx<R>{a.x * s, a.y * s};
+}
+// Multiply by S*i with S = +1 or -1 known at compile time.
+template <int S, typename R>
+__host__ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
+  if constexpr (S > 0) {
+    return cpx<R>{-a.y, a.x};
+  } else {
+    return cpx<R>{a.y, -a.x};
+  }
+}
+
+// ------------------------------------------------------------ static loop --
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// --------------------------------------------------- compile-time sin/cos --
+// Taylor series on |a| <= pi/4 after an exact integer octant reduction of the
+// angle 2*pi*r/m.  Used only in constant expressions (codelet twiddles).
+constexpr double ce_sin_small(double a) {
+  double term = a, sum = a;
+  for (int n = 1; n < 15; ++n) {
+    term *= -a * a / double((2 * n) * (2 * n + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_cos_small(double a) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 15; ++n) {
+    term *= -a * a / double((2 * n - 1) * (2 * n));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double kQuarterPi = 0.78539816339744830961566084581987572;
+
+// cos(2*pi*r/m) (want_sin == false) or sin(2*pi*r/m) (want_sin == true).
+constexpr double ce_trig2pi(long r, long m, bool want_sin) {
+  long rr = r % m;
+  if (rr < 0) rr += m;
+  const long u = 8 * rr;        // angle = (pi/4) * u / m
+  const long oct = u / m;       // 0..7
+  const long f = u % m;         // remainder within the octant
+  const double a = kQuarterPi * double(f) / double(m);
+  const double b = kQuarterPi * double(m - f) / double(m);
+  double s = 0.0, c = 0.0;
+  switch (oct) {
+    case 0: s = ce_sin_small(a); c = ce_cos_small(a); break;
+    case 1: s = ce_cos_small(b); c = ce_sin_small(b); break;
+    case 2: s = ce_cos_small(a); c = -ce_sin_small(a); break;
+    case 3: s = ce_sin_small(b); c = -ce_cos_small(b); break;
+    case 4: s = -ce_sin_small(a); c = -ce_cos_small(a); break;
+    case 5: s = -ce_cos_small(b); c = -ce_sin_small(b); break;
+    case 6: s = -ce_cos_small(a); c = ce_sin_small(a); break;
+    default: s = -ce_sin_small(b); c = ce_cos_small(b); break;
+  }
+  return want_sin ? s : c;
+}
+
+constexpr int ce_first_factor(int m) {
+  if (m % 4 == 0 && m != 4) return 4;
+  if (m % 2 == 0 && m != 2) return 2;
+  if (m % 3 == 0 && m != 3) return 3;
+  if (m % 5 == 0 && m != 5) return 5;
+  if (m % 7 == 0 && m != 7) return 7;
+  return m;  // prime (or a base case)
+}
+
+// a *= W_m^r with W_m = exp(DIR * 2*pi*i / m); DIR = -1 forward, +1 inverse.
+template <int r, int m, int DIR, typename R>
+__device__ __forceinline__ void twiddle(cpx<R>& a) {
+  constexpr int rr = ((r % m) + m) % m;
+  if constexpr (rr == 0) {
+    return;
+  } else if constexpr (2 * rr == m) {
+    a = cpx<R>{-a.x, -a.y};
+  } else if constexpr (4 * rr == m) {
+    a = mul_i<DIR>(a);
+  } else if constexpr (4 * rr == 3 * m) {
+    a = mul_i<-DIR>(a);
+  } else if constexpr ((8 * rr) % m == 0) {
+    // odd multiples of pi/4: (c + DIR*i*s) with |c| = |s| = sqrt(1/2)
+    constexpr double c = ce_trig2pi(rr, m, false);
+    constexpr double s = DIR * ce_trig2pi(rr, m, true);
+    constexpr R h = R(0.70710678118654752440084436210484904);
+    constexpr R sc = c > 0 ? h : -h;
+    constexpr R ss = s > 0 ? h : -h;
+    // (x + iy)(sc + i ss) with |sc| = |ss| = h
+    const R p = a.x, q = a.y;
+    if constexpr (sc > 0 && ss > 0) {
+      a = cpx<R>{(p - q) * h, (p + q) * h};
+    } else if constexpr (sc > 0 && ss < 0) {
+      a = cpx<R>{(p + q) * h, (q - p) * h};
+    } else if constexpr (sc < 0 && ss > 0) {
+      a = cpx<R>{-(p + q) * h, (p - q) * h};
+    } else {
+      a = cpx<R>{(q - p) * h, -(p + q) * h};
+    }
+  } else {
+    constexpr R c = R(ce_trig2pi(rr, m, false));
+    constexpr R s = R(DIR * ce_trig2pi(rr, m, true));
+    a = cpx<R>{a.x * c - a.y * s, a.x * s + a.y * c};
+  }
+}
+
+// ---------------------------------------------------------------- codelets --
+// In-place DFT of length M held in registers, natural order in and out:
+//   v[k] <- sum_n v[n] * exp(DIR * 2*pi*i*n*k / M)
+template <int M, int DIR, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v);
+
+template <int M, int DIR, typename R>
+__device__ __forceinline__ void dft_prime(cpx<R>* v) {
+  cpx<R> out[M];
+  static_for<0, M>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    cpx<R> acc = v[0];
+    static_for<1, M>([&](auto nc) {
+      constexpr int n = decltype(nc)::value;
+      cpx<R> term = v[n];
+      twiddle<(n * k) % M, M, DIR>(term);
+      acc = acc + term;
+    });
+    out[k] = acc;
+  });
+  static_for<0, M>([&](auto kc) { v[decltype(kc)::value] = out[decltype(kc)::value]; });
+}
+
+template <int M, int DIR, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (M == 1) {
+    return;
+  } else if constexpr (M == 2) {
+    const cpx<R> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (M == 3) {
+    constexpr R h = R(0.86602540378443864676372317075293618);  // sqrt(3)/2
+    const cpx<R> a0 = v[0], t = v[1] + v[2], d = v[1] - v[2];
+    const cpx<R> m = cpx<R>{a0.x - R(0.5) * t.x, a0.y - R(0.5) * t.y};
+    const cpx<R> r = mul_i<DIR>(cscale(d, h));
+    v[0] = a0 + t;
+    v[1] = m + r;
+    v[2] = m - r;
+  } else if constexpr (M == 4) {
+    const cpx<R> s02 = v[0] + v[2], d02 = v[0] - v[2];
+    const cpx<R> s13 = v[1] + v[3], d13 = mul_i<DIR>(v[1] - v[3]);
+    v[0] = s02 + s13;
+    v[2] = s02 - s13;
+    v[1] = d02 + d13;
+    v[3] = d02 - d13;
+  } else if constexpr (ce_first_factor(M) == M) {
+    dft_prime<M, DIR, R>(v);
+  } else {
+    // Cooley-Tukey M = P*Q:  n = Q*n1 + n2,  k = k1 + P*k2.
+    constexpr int P = ce_first_factor(M);
+    constexpr int Q = M / P;
+    cpx<R> a[Q][P];
+    static_for<0, Q>([&](auto n2c) {
+      constexpr int n2 = decltype(n2c)::value;
+      static_for<0, P>([&](auto n1c) {
+        constexpr int n1 = decltype(n1c)::value;
+        a[n2][n1] = v[Q * n1 + n2];
+      });
+      dft<P, DIR, R>(a[n2]);
+      static_for<1, P>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        twiddle<n2 * k1, M, DIR>(a[n2][k1]);
+      });
+    });
+    static_for<0, P>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      cpx<R> b[Q];
+      static_for<0, Q>([&](auto n2c) { b[decltype(n2c)::value] = a[decltype(n2c)::value][k1]; });
+      dft<Q, DIR, R>(b);
+      static_for<0, Q>([&](auto k2c) {
+        constexpr int k2 = decltype(k2c)::value;
+        v[k1 + P * k2] = b[k2];
+      });
+    });
+  }
+}
+
+// ------------------------------------------------------------- reductions --
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide deterministic sum of NQ doubles; result valid in thread 0.
+// `scratch` must hold NQ * (blockDim.x / 64) doubles.
+template <int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) scratch[q * nw + wave] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += scratch[q * nw + w];
+      v[q] = s;
+    }
+  }
+}
+
+__device__ __forceinline__ double coherent_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace gpe

@@ -1,0 +1,57 @@
+// Host-side declarations of the launchers in gpe_misc.hip and the pass
+// launchers compiled from gpe_pass_inst.hip.
+#pragma once
+
+#include "gpe_passes.h"
+#include "gpe_vm.h"
+
+namespace gpe {
+
+constexpr int kNormBlocks = 1024;
+
+template <typename R> void launch_twiddle(void* tw, int n, hipStream_t st);
+template <typename R>
+void launch_kinetic(void* kx, void* ky, void* kz, const double* karr, int n, double dt, double coef, int imag,
+                    double inv_n3, hipStream_t st);
+void launch_vm_static(const int* code, int ncode, const double* consts, int target, int n, const double* xa,
+                      long long size, double scale, double* master, hipStream_t st);
+template <typename R>
+void launch_vm_dynamic(const int* code, int ncode, const double* consts, int target, int n, const double* xa,
+                       long long size, const double* master, const double* tlist, int nslots, void* out,
+                       hipStream_t st);
+template <typename R> void launch_cast(const double* in, void* out, long long size, hipStream_t st);
+void launch_fill_f64(double* out, long long size, double v, hipStream_t st);
+void launch_add_f64(double* out, const double* in, long long size, hipStream_t st);
+template <typename R> void launch_scale(void* psi, long long count, const double* scal, hipStream_t st);
+template <typename R> void launch_gaussian(void* psi, const double* xa, int n, double sigma, hipStream_t st);
+template <typename R> void launch_from_c128(const void* in, void* out, long long count, hipStream_t st);
+template <typename R> void launch_to_c128(const void* in, void* out, long long count, hipStream_t st);
+template <typename R>
+void launch_norm(const void* psi, long long count, double* red, unsigned int* counter, double* scal, double dx3,
+                 int prescale, hipStream_t st);
+void launch_finish_sums(const double* red, long long nblk, int nq, double* out, hipStream_t st);
+void launch_finish_nk(const double* occ, const unsigned int* cnt, long long nblk, int nbins, double* out_occ,
+                      long long* out_cnt, hipStream_t st);
+void launch_copy(const void* in, void* out, long long bytes, hipStream_t st);
+
+// Pass launchers: one per (precision, side) instantiation; side 0 = generic.
+using PassLauncher = hipError_t (*)(int kind, const void* args, int n, int batch, hipStream_t st);
+using PassBlocks = long long (*)(int kind, int n);
+
+#define GPE_DECL_PASS(P, N)                                                              \
+  hipError_t launch_pass_p##P##_n##N(int kind, const void* args, int n, int batch, hipStream_t st); \
+  long long pass_blocks_p##P##_n##N(int kind, int n);
+GPE_DECL_PASS(0, 0)
+GPE_DECL_PASS(0, 64)
+GPE_DECL_PASS(0, 96)
+GPE_DECL_PASS(0, 128)
+GPE_DECL_PASS(0, 192)
+GPE_DECL_PASS(0, 256)
+GPE_DECL_PASS(1, 0)
+GPE_DECL_PASS(1, 64)
+GPE_DECL_PASS(1, 96)
+GPE_DECL_PASS(1, 128)
+GPE_DECL_PASS(1, 192)
+GPE_DECL_PASS(1, 256)
+
+}  // namespace gpe

@@ -1,0 +1,477 @@
+// Pencil passes of the split-step propagator.
+//
+// A 3D field psi[i][j][k] (C order, z = k fastest, n^3 points) is transformed
+// one axis at a time.  A "pass" is one read + one write sweep of the field in
+// which every block owns a tile of whole pencils along one axis, runs a 1D FFT
+// (or FFT -> pointwise -> inverse FFT) on them and fuses the pointwise physics:
+//
+//   Z passes (contiguous axis): local half-steps exp(-i(V + beta|psi|^2)dt/2),
+//                               renormalisation scale, FP64 norm / energy sums;
+//   Y passes (stride n):        plain forward / inverse FFT;
+//   X pass  (stride n^2):       FFT_x -> kinetic factor -> inverse FFT_x.
+//
+// Two pencil transforms share the same pass code:
+//   FastTF<R, N, TP, B>: N = TP * M with TP threads per pencil; each thread
+//     holds M values at positions t + TP*m; a length-M codelet in registers,
+//     an FP64-derived twiddle, one LDS exchange and TP-point codelets.  The
+//     output comes back in the same register layout, so FFT -> pointwise ->
+//     inverse FFT needs no extra shuffles.
+//   GenTF<R>: any n <= 1024, one pencil per 64-thread block, direct DFT from
+//     LDS with an exact (n*k mod n) twiddle index.  Used for sizes that have
+//     no fast instantiation.
+#pragma once
+
+#include "gpe_device.h"
+
+namespace gpe {
+
+enum PassKind : int {
+  K_Z_FIRST = 0,   // [scale] -> half(slot_a) -> FFT_z
+  K_Z_MID = 1,     // IFFT_z -> half(slot_a) -> half(slot_b) -> FFT_z   (real time)
+  K_Z_LAST = 2,    // IFFT_z -> half(slot_a) [-> FP64 norm -> scale]
+  K_Y_FWD = 3,
+  K_Y_INV = 4,
+  K_X_KIN = 5,     // FFT_x -> kinetic -> IFFT_x
+  K_Z_DIAG = 6,    // [scale] -> FP64 sums in real space -> FFT_z into work
+  K_X_DIAG = 7,    // FFT_x of work -> FP64 kinetic sums and |psi(k)|^2 shells
+  K_Z_FWD = 8,     // plain FFT_z (psi -> work)
+  K_Y_LZ = 9,      // FFT_y, FP64 sum x * k_y * |F|^2 (no write)
+  K_X_LZ = 10,     // FFT_x, FP64 sum y * k_x * |F|^2 (no write)
+};
+
+enum PassFlags : int {
+  F_PRESCALE = 1,  // multiply the loaded field by scal[1] (pending 1/sqrt(norm))
+  F_IN_WORK = 2,   // read from the workspace instead of psi
+  F_OUT_WORK = 4,  // write to the workspace instead of psi
+  F_NORM = 8,      // Z_LAST: FP64 norm reduction, last block computes scale
+};
+
+// Per-solve device descriptor.  Lives in device memory; kernels index it with
+// blockIdx.z so equal-shape solves can share one launch.
+template <typename R>
+struct SolveDev {
+  cpx<R>* psi;
+  cpx<R>* work;
+  const cpx<R>* tw;        // tw[r] = exp(-2*pi*i*r/n), r < n (FP64-derived)
+  const cpx<R>* kin_x;     // kinetic factors per axis (1/n^3 folded into x)
+  const cpx<R>* kin_y;
+  const cpx<R>* kin_z;
+  const double* karr;      // k axis, FP64
+  const double* xarr;      // x axis, FP64
+  const R* pxy;            // potential table over (x, y): [slot][n*n]
+  const R* pz;             // potential table over z: [slot][n]
+  const R* vre;            // static volume, real part (nullable)
+  const R* vim;            // static volume, imaginary part (nullable)
+  const R* vdyn;           // time-dependent volume [slot][n^3] (nullable)
+  long long pxy_ss, pz_ss, vdyn_ss;   // slot strides (0 = static table)
+  double* red;             // per-block partial sums
+  unsigned int* counter;   // last-block ticket (self-resetting)
+  double* scal;            // [0] norm, [1] 1/sqrt(norm)
+  double* hist;            // norm history per step (nullable)
+  double* nkocc;           // [block][nbins] |F|^2 partials
+  unsigned int* nkcnt;     // [block][nbins] mode-count partials
+  double dx3;              // dx*dx*dx
+  double kcut2;            // k_cut^2 (masked kinetic)
+  R beta;
+  R hdt;                   // dt/2
+  int n;
+  int nbins;
+  int imag;                // 1: imaginary time
+  int masked;              // 1: zero modes with k^2 > kcut^2
+};
+
+template <typename R>
+struct PassArgs {
+  const SolveDev<R>* s;
+  int slot_a, slot_b;
+  int flags;
+  int hist_index;          // step index into s->hist (Z_LAST with F_NORM)
+};
+
+// ------------------------------------------------------------ transforms --
+template <typename R, int N_, int TP_, int B_>
+struct FastTF {
+  static constexpr bool kFast = true;
+  static constexpr int N = N_, TP = TP_, B = B_;
+  static constexpr int M = N / TP;
+  static constexpr int Q = M / TP;
+  static constexpr int THREADS = TP * B;
+  static constexpr int LSTRIDE = TP + 1;           // padded LDS row
+  static constexpr int PS = M * LSTRIDE;           // LDS elements per pencil
+  static constexpr int LDS_ELEMS = B * PS;
+  static_assert(N % TP == 0 && M % TP == 0, "N must be TP*TP*Q");
+
+  __device__ static __forceinline__ int pos(int t, int m) { return t + TP * m; }
+  __device__ static __forceinline__ bool valid(int, int, int) { return true; }
+
+  template <int DIR>
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* __restrict__ tw,
+                                             int) {
+    dft<M, DIR, R>(v);
+    static_for<1, M>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      cpx<R> w = tw[t * k1];
+      if constexpr (DIR > 0) w.y = -w.y;
+      v[k1] = cmul(v[k1], w);
+    });
+    __syncthreads();
+    static_for<0, M>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      lp[k1 * LSTRIDE + t] = v[k1];
+    });
+    __syncthreads();
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const int k1 = t + TP * q;
+      cpx<R> u[TP];
+      static_for<0, TP>([&](auto sc) { u[decltype(sc)::value] = lp[k1 * LSTRIDE + decltype(sc)::value]; });
+      dft<TP, DIR, R>(u);
+      static_for<0, TP>([&](auto k2c) {
+        constexpr int k2 = decltype(k2c)::value;
+        v[q + Q * k2] = u[k2];
+      });
+    });
+  }
+};
+
+template <typename R>
+struct GenTF {
+  static constexpr bool kFast = false;
+  static constexpr int N = 0, TP = 64, B = 1, M = 16;
+  static constexpr int THREADS = 64;
+  static constexpr int PS = 1024;
+  static constexpr int LDS_ELEMS = 1024;
+
+  __device__ static __forceinline__ int pos(int t, int m) { return t + 64 * m; }
+  __device__ static __forceinline__ bool valid(int t, int m, int n) { return t + 64 * m < n; }
+
+  template <int DIR>
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* __restrict__ tw,
+                                             int n) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+      if (t + 64 * m < n) lp[t + 64 * m] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int k = t + 64 * m;
+      if (k < n) {
+        cpx<R> acc{R(0), R(0)};
+        int idx = 0;
+        for (int s = 0; s < n; ++s) {
+          cpx<R> w = tw[idx];
+          if constexpr (DIR > 0) w.y = -w.y;
+          acc = acc + cmul(lp[s], w);
+          idx += k;
+          if (idx >= n) idx -= n;
+        }
+        v[m] = acc;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------- pointwise --
+template <typename R>
+__device__ __forceinline__ void fast_sincos(R x, R* s, R* c);
+template <>
+__device__ __forceinline__ void fast_sincos<float>(float x, float* s, float* c) {
+  sincosf(x, s, c);
+}
+template <>
+__device__ __forceinline__ void fast_sincos<double>(double x, double* s, double* c) {
+  sincos(x, s, c);
+}
+__device__ __forceinline__ float rexp(float x) { return expf(x); }
+__device__ __forceinline__ double rexp(double x) { return exp(x); }
+
+// One local half-step at a point with total external potential `pot`
+// (static + dynamic, real part) and static imaginary part `vim`.
+template <typename R>
+__device__ __forceinline__ cpx<R> half_step(cpx<R> v, R pot, R vim, bool has_vim, const SolveDev<R>& s) {
+  const R a2 = v.x * v.x + v.y * v.y;
+  const R e = (pot + s.beta * a2) * s.hdt;
+  if (s.imag) {
+    const R f = rexp(-e);
+    return cpx<R>{v.x * f, v.y * f};
+  }
+  R sn, cs;
+  fast_sincos<R>(e, &sn, &cs);
+  cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+  if (has_vim) {
+    const R d = rexp(vim * s.hdt);
+    r = cpx<R>{r.x * d, r.y * d};
+  }
+  return r;
+}
+
+__device__ __forceinline__ int freq_index(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+
+// Shell index floor(|k|/dk + 1/2) from integer frequencies: with m = isqrt(q),
+// sqrt(q) rounds up exactly when q > m^2 + m (sqrt(q) never equals m + 1/2).
+__device__ __forceinline__ int shell_index(int p, int q, int r) {
+  const int s = p * p + q * q + r * r;
+  int m = int(sqrtf(float(s)));
+  while (m * m > s) --m;
+  while ((m + 1) * (m + 1) <= s) ++m;
+  return m + (s > m * m + m ? 1 : 0);
+}
+
+// ------------------------------------------------------------ pass kernel --
+template <class TF, int AXIS, int KIND, typename R>
+__global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
+  __shared__ cpx<R> lds[TF::LDS_ELEMS];
+  const SolveDev<R>& s = a.s[blockIdx.z];
+  const int n = TF::kFast ? TF::N : s.n;
+  constexpr int M = TF::M;
+
+  int t, b;
+  if constexpr (AXIS == 2) {
+    t = threadIdx.x % TF::TP;
+    b = threadIdx.x / TF::TP;
+  } else {
+    b = threadIdx.x % TF::B;
+    t = threadIdx.x / TF::B;
+  }
+  long long base, stride;
+  int pi = 0, pj = 0, pk = 0;  // pencil coordinates (axis coordinate unused)
+  if constexpr (AXIS == 2) {
+    const long long row = (long long)blockIdx.x * TF::B + b;
+    pi = int(row / n);
+    pj = int(row % n);
+    base = row * n;
+    stride = 1;
+  } else if constexpr (AXIS == 1) {
+    pi = blockIdx.y;
+    pk = blockIdx.x * TF::B + b;
+    base = (long long)pi * n * n + pk;
+    stride = n;
+  } else {
+    pj = blockIdx.y;
+    pk = blockIdx.x * TF::B + b;
+    base = (long long)pj * n + pk;
+    stride = (long long)n * n;
+  }
+  cpx<R>* lp = lds + b * TF::PS;
+  const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
+  cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
+
+  cpx<R> v[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (TF::valid(t, m, n)) v[m] = src[base + (long long)TF::pos(t, m) * stride];
+  }
+
+  if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD) {
+    TF::template fft<-1>(v, lp, t, s.tw, n);
+  } else if constexpr (KIND == K_Y_INV) {
+    TF::template fft<+1>(v, lp, t, s.tw, n);
+  } else if constexpr (KIND == K_X_KIN) {
+    TF::template fft<-1>(v, lp, t, s.tw, n);
+    if (s.imag) {
+      const R eyz = s.kin_y[pj].x * s.kin_z[pk].x;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const R f = s.kin_x[TF::pos(t, m)].x * eyz;
+          v[m] = cpx<R>{v[m].x * f, v[m].y * f};
+        }
+      }
+    } else {
+      const cpx<R> eyz = cmul(s.kin_y[pj], s.kin_z[pk]);
+      double ky2 = 0.0, kz2 = 0.0;
+      if (s.masked) {
+        ky2 = __dmul_rn(s.karr[pj], s.karr[pj]);
+        kz2 = __dmul_rn(s.karr[pk], s.karr[pk]);
+      }
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const int i = TF::pos(t, m);
+          cpx<R> f = cmul(s.kin_x[i], eyz);
+          if (s.masked) {
+            const double kx = s.karr[i];
+            // same association as kx*kx + ky*ky + kz*kz, without contraction
+            const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
+            if (k2 > s.kcut2) f = cpx<R>{R(0), R(0)};
+          }
+          v[m] = cmul(v[m], f);
+        }
+      }
+    }
+    TF::template fft<+1>(v, lp, t, s.tw, n);
+  } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST) {
+    if constexpr (KIND != K_Z_FIRST) {
+      TF::template fft<+1>(v, lp, t, s.tw, n);
+    } else {
+      if (a.flags & F_PRESCALE) {
+        const R sc = R(s.scal[1]);
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+      }
+    }
+    const long long rowoff = ((long long)pi * n + pj) * n;
+    const R pxa = s.pxy[a.slot_a * s.pxy_ss + (long long)pi * n + pj];
+    R pxb = R(0);
+    if constexpr (KIND == K_Z_MID) pxb = s.pxy[a.slot_b * s.pxy_ss + (long long)pi * n + pj];
+    const bool has_vim = s.vim != nullptr;
+    double nsum = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (TF::valid(t, m, n)) {
+        const int k = TF::pos(t, m);
+        R vst = R(0), vi = R(0);
+        if (s.vre) vst = s.vre[rowoff + k];
+        if (has_vim) vi = s.vim[rowoff + k];
+        R pa = pxa + s.pz[a.slot_a * s.pz_ss + k] + vst;
+        if (s.vdyn) pa += s.vdyn[a.slot_a * s.vdyn_ss + rowoff + k];
+        cpx<R> w = half_step(v[m], pa, vi, has_vim, s);
+        if constexpr (KIND == K_Z_MID) {
+          R pb = pxb + s.pz[a.slot_b * s.pz_ss + k] + vst;
+          if (s.vdyn) pb += s.vdyn[a.slot_b * s.vdyn_ss + rowoff + k];
+          w = half_step(w, pb, vi, has_vim, s);
+        }
+        if constexpr (KIND == K_Z_LAST) {
+          nsum += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+        }
+        v[m] = w;
+      }
+    }
+    if constexpr (KIND != K_Z_LAST) {
+      TF::template fft<-1>(v, lp, t, s.tw, n);
+    } else {
+      if (a.flags & F_NORM) {
+        // FP64 block partial, then the last block to finish sums all
+        // partials in a fixed order and publishes norm and 1/sqrt(norm).
+        __shared__ double red_scratch[16];
+        __shared__ int is_last;
+        double acc[1] = {nsum};
+        block_sum<1>(acc, red_scratch);
+        const unsigned nblk = gridDim.x * gridDim.y;
+        const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
+        if (threadIdx.x == 0) {
+          s.red[bid] = acc[0];
+          __threadfence();
+          const unsigned ticket = atomicAdd(s.counter, 1u);
+          is_last = (ticket == nblk - 1);
+        }
+        __syncthreads();
+        if (is_last) {
+          __threadfence();
+          double part[1] = {0.0};
+          for (unsigned q = threadIdx.x; q < nblk; q += blockDim.x) part[0] += coherent_load(s.red + q);
+          block_sum<1>(part, red_scratch);
+          if (threadIdx.x == 0) {
+            const double norm = s.dx3 * part[0];
+            s.scal[0] = norm;
+            s.scal[1] = 1.0 / sqrt(norm);
+            if (s.hist) s.hist[a.hist_index] = norm;
+            *s.counter = 0u;
+          }
+        }
+      }
+    }
+  } else if constexpr (KIND == K_Z_DIAG) {
+    // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
+    if (a.flags & F_PRESCALE) {
+      const R sc = R(s.scal[1]);
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+    }
+    const long long rowoff = ((long long)pi * n + pj) * n;
+    const double pxa = double(s.pxy[a.slot_a * s.pxy_ss + (long long)pi * n + pj]);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (TF::valid(t, m, n)) {
+        const int k = TF::pos(t, m);
+        double pot = pxa + double(s.pz[a.slot_a * s.pz_ss + k]);
+        if (s.vre) pot += double(s.vre[rowoff + k]);
+        if (s.vdyn) pot += double(s.vdyn[a.slot_a * s.vdyn_ss + rowoff + k]);
+        const double a2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+        acc[0] += a2;
+        acc[1] += pot * a2;
+        acc[2] += a2 * a2;
+      }
+    }
+    __shared__ double red_scratch[48];
+    block_sum<3>(acc, red_scratch);
+    if (threadIdx.x == 0) {
+      const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
+      s.red[bid * 3 + 0] = acc[0];
+      s.red[bid * 3 + 1] = acc[1];
+      s.red[bid * 3 + 2] = acc[2];
+    }
+    TF::template fft<-1>(v, lp, t, s.tw, n);
+  } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
+    // Angular momentum by Parseval along one axis:
+    //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
+    if (a.flags & F_PRESCALE) {
+      const R sc = R(s.scal[1]);
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+    }
+    TF::template fft<-1>(v, lp, t, s.tw, n);
+    const double coord = KIND == K_Y_LZ ? s.xarr[pi] : s.xarr[pj];
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (TF::valid(t, m, n)) {
+        const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+        acc[0] += s.karr[TF::pos(t, m)] * f2;
+      }
+    }
+    acc[0] *= coord;
+    __shared__ double red_scratch[16];
+    block_sum<1>(acc, red_scratch);
+    if (threadIdx.x == 0) s.red[blockIdx.x + gridDim.x * blockIdx.y] = acc[0];
+    return;
+  } else if constexpr (KIND == K_X_DIAG) {
+    TF::template fft<-1>(v, lp, t, s.tw, n);
+    __shared__ double hocc[1024];
+    __shared__ unsigned int hcnt[1024];
+    for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
+      hocc[q] = 0.0;
+      hcnt[q] = 0u;
+    }
+    __syncthreads();
+    const double ky = s.karr[pj], kz = s.karr[pk];
+    const int fq = freq_index(pj, n), fr = freq_index(pk, n);
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (TF::valid(t, m, n)) {
+        const int i = TF::pos(t, m);
+        const double kx = s.karr[i];
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+        acc[0] += f2 * k2;
+        acc[1] += f2;
+        const int sh = shell_index(freq_index(i, n), fq, fr);
+        atomicAdd(&hocc[sh], f2);
+        atomicAdd(&hcnt[sh], 1u);
+      }
+    }
+    __shared__ double red_scratch[32];
+    block_sum<2>(acc, red_scratch);
+    const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
+    if (threadIdx.x == 0) {
+      s.red[bid * 2 + 0] = acc[0];
+      s.red[bid * 2 + 1] = acc[1];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
+      s.nkocc[(long long)bid * s.nbins + q] = hocc[q];
+      s.nkcnt[(long long)bid * s.nbins + q] = hcnt[q];
+    }
+    return;
+  }
+
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (TF::valid(t, m, n)) dst[base + (long long)TF::pos(t, m) * stride] = v[m];
+  }
+}
+
+}  // namespace gpe

@@ -1,0 +1,165 @@
+/* libgpe_hip.so — C ABI of the MI355X split-step Gross-Pitaevskii engine.
+ *
+ * Plain C types only: opaque handles, host pointers and sizes.  Every call
+ * returns a status (0 = ok, negative = error; gpe_last_error() describes the
+ * most recent failure of the calling thread).  The library owns all device
+ * memory; host pointers are borrowed for the duration of a call and never
+ * retained.  Each solve owns one HIP stream; calls on one solve are ordered,
+ * calls on different solves may overlap on the device.
+ *
+ * Equation (reference units, PoyntingZhang/GPE_simulator):
+ *     i dpsi/dt = [-coef*lap + V + beta*|psi|^2] psi      (coef = 1 in the reference)
+ * Strang step: local half-step -> 3D FFT -> kinetic -> inverse FFT ->
+ * local half-step; imaginary time renormalises after every step.
+ * Reference citations are paths inside the reference repository.
+ */
+#ifndef GPE_HIP_H
+#define GPE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gpe_ctx gpe_ctx;
+typedef struct gpe_solve gpe_solve;
+
+enum {
+  GPE_OK = 0,
+  GPE_ERR_INVALID = -1,   /* bad argument or contract violation (reference: AssertionError) */
+  GPE_ERR_NOMEM = -2,     /* device allocation failed */
+  GPE_ERR_HIP = -3,       /* HIP runtime error */
+  GPE_ERR_STATE = -4      /* call not valid in the solve's current state */
+};
+
+enum { GPE_C64 = 0, GPE_C128 = 1 };            /* field precision */
+enum { GPE_REAL = 0, GPE_IMAG = 1 };           /* evolution kind */
+enum { GPE_TGT_XY = 0, GPE_TGT_Z = 1, GPE_TGT_VOL = 2, GPE_TGT_VOL_IM = 3 };
+
+/* Library version string. */
+const char* gpe_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gpe_last_error(void);
+
+/* Context on one HIP device.  No reference counterpart (CuPy's current
+ * device, gpe/main.py:27-38 runs on it implicitly). */
+int gpe_ctx_create(int device, gpe_ctx** out);
+int gpe_ctx_destroy(gpe_ctx* ctx);
+/* Free / total device memory in bytes (serving admission). */
+int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* Solve description.  Replaces what make_real_propagator / make_imag_propagator
+ * (gpe/make_propagator.py:5, :106) read from `stage` (stage.beta, stage.dt,
+ * stage.N_grids, stage.x_array, stage.k_array, stage.k_cut_params;
+ * gpe/stage.py:11-51). */
+typedef struct gpe_solve_desc {
+  int32_t n;                 /* grid side N (N^3 points), 2..1024            */
+  int32_t precision;         /* GPE_C64 or GPE_C128                           */
+  int32_t kind;              /* GPE_REAL or GPE_IMAG                          */
+  int32_t kcut_enable;       /* masked kinetic step (gpe/kernels.py:24-45)    */
+  double dt;                 /* time step (reference units)                   */
+  double beta;               /* interaction strength (gpe/helper.py:109-113)  */
+  double kcut;               /* cutoff wavenumber when kcut_enable            */
+  double kinetic_coef;       /* 1.0 reproduces the reference's -lap           */
+  const double* x_array;     /* host, n values (gpe/grids.py:3-6)             */
+  const double* k_array;     /* host, n values (gpe/grids.py:3-6)             */
+} gpe_solve_desc;
+
+/* Allocate psi (N^3 complex of the requested precision) and the step tables
+ * on the device.  Reference: stage.prepare building the propagator
+ * (gpe/stage.py:53-66, gpe/make_propagator.py:5-152). */
+int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* desc, gpe_solve** out);
+int gpe_solve_destroy(gpe_solve* solve);
+/* Device bytes a solve of this description holds (psi + tables), and the
+ * extra bytes the diagnostics workspace adds when first used. */
+int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size_t* diag_bytes);
+
+/* Copy elements [offset, offset+count) of psi (flat C-order index) from / to
+ * a host complex128 buffer (interleaved re, im).  complex64 solves convert
+ * on the device.  Reference: stage.psi assignment / cp.asnumpy and the slab
+ * copies of gpe/state_io.py:14-58. */
+int gpe_psi_upload(gpe_solve* solve, const double* host_c128, int64_t offset, int64_t count);
+int gpe_psi_download(gpe_solve* solve, double* host_c128, int64_t offset, int64_t count);
+/* dst.psi = src.psi on the device (same side; precisions may differ).
+ * Reference: a `continue` stage reusing the previous stage's array
+ * (gpe/stage.py:74-79, gpe/main.py:31-36). */
+int gpe_psi_copy(gpe_solve* dst, gpe_solve* src);
+/* psi = separable Gaussian exp(-r^2/(2 sigma^2)), then normalised.
+ * Reference: gpe/initial_states.py:4-14. */
+int gpe_psi_gaussian(gpe_solve* solve, double sigma);
+
+/* Potential programs (postfix code, see gpe_vm.h in the sources).
+ * gpe_potential_static adds scale * f(x, y, z) to the FP64 table of `target`
+ * (evaluated once, on the device).  gpe_potential_dynamic registers f(x, y, z, t)
+ * for `target` (GPE_TGT_XY, GPE_TGT_Z or GPE_TGT_VOL), evaluated in FP64 at
+ * every local half-step time t_mid.  Replaces the expression splicing of
+ * make_real_spatial_kernel / make_imag_spatial_kernel (gpe/make_kernel.py:4-120). */
+int gpe_potential_static(gpe_solve* solve, int32_t target, const int32_t* code, int32_t ncode,
+                         const double* consts, int32_t nconst, double scale);
+int gpe_potential_dynamic(gpe_solve* solve, int32_t target, const int32_t* code, int32_t ncode,
+                          const double* consts, int32_t nconst);
+/* Static volume V_static = re + i*im (target GPE_TGT_VOL / GPE_TGT_VOL_IM):
+ * fill with a constant, or upload a slab of host float64 values that replace
+ * (accumulate = 0) or are added to (accumulate = 1) elements [offset,
+ * offset+count).  Reference: stage.staticV (gpe/stage.py:56-62,
+ * gpe/potentials.py:31-61). */
+int gpe_volume_fill(gpe_solve* solve, int32_t target, double value);
+int gpe_volume_upload(gpe_solve* solve, int32_t target, const double* host, int64_t offset, int64_t count,
+                      int32_t accumulate);
+
+/* Advance nsteps Strang steps.  t_mid holds the FP64 midpoint time of each
+ * step (t_i + dt/2, as gpe/make_propagator.py:94); it may be NULL when no
+ * time-dependent program is registered.  Consecutive steps inside one call
+ * fuse the second half-step of step i with the first of step i+1.
+ * Reference: the propagator closures (gpe/make_propagator.py:12-54, 57-103,
+ * 109-151) called once per step by launch_stage (gpe/launch.py:8-12). */
+int gpe_step(gpe_solve* solve, int64_t nsteps, const double* t_mid);
+/* Norm per step of the last imaginary-time gpe_step call (before
+ * renormalisation), `count` values ending with the last step. */
+int gpe_norm_history(gpe_solve* solve, double* out, int64_t count);
+
+/* FP64 norm dx^3 * sum |psi|^2.  Reference: get_N (gpe/helper.py:81-88). */
+int gpe_norm(gpe_solve* solve, double* out);
+/* psi /= sqrt(norm).  Reference: normalise (gpe/helper.py:91-93). */
+int gpe_normalise(gpe_solve* solve);
+
+/* All diagnostics from one forward FFT of the completed psi, FP64 sums:
+ *   scalars[0] norm      dx^3 sum |psi|^2                 (get_N)
+ *   scalars[1] E_kin     dx^3/N^3 sum k^2 |F|^2           (get_E, gpe/helper.py:135-140)
+ *   scalars[2] E_pot     dx^3 sum V(t)|psi|^2
+ *   scalars[3] E_int     beta/2 dx^3 sum |psi|^4
+ *   scalars[4] E         E_kin + E_pot + E_int
+ *   scalars[5] mu        (E_kin + E_pot + 2 E_int) / norm
+ *   scalars[6] parseval  dx^3/N^3 sum |F|^2
+ * counts / occupation: per shell floor(|k|/dk + 1/2), nbins values, raw
+ * sum |F|^2 (multiply by dx^3/N^3 for Nk).  Reference: save_outputs'
+ * shared fftn (gpe/save.py:31-57) and nk_extraction (gpe/nk_extraction.py:16-45).
+ * t is the time at which the time-dependent potential enters E_pot. */
+int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts, double* occupation,
+                    int32_t nbins);
+
+/* Lz = Re int psi^* (x p_y - y p_x) psi dV with spectral derivatives, FP64.
+ * Reference: get_L (gpe/helper.py:143-156). */
+int gpe_angular_momentum(gpe_solve* solve, double* out);
+
+/* Wait for all work of the solve. */
+int gpe_sync(gpe_solve* solve);
+
+/* Measurement hooks (no reference counterpart; gpe/benchmark.py:59-69 times
+ * whole batches).  gpe_step_timed runs gpe_step with HIP events around every
+ * pass launch on the solve's stream and reports the summed milliseconds per
+ * pass kind (9 kinds, see GPE_PASS_* in the sources) and launch counts. */
+int gpe_step_timed(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* ms_per_kind,
+                   int64_t* launches_per_kind);
+/* Device-to-device streaming copy of `bytes` (multiple of 16), repeated
+ * `reps` times; returns milliseconds per copy.  Bandwidth calibration. */
+int gpe_bench_copy(gpe_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPE_HIP_H */

@@ -1,0 +1,135 @@
+"""GPU parity of libgpe_hip against the FP64 oracle on identical seeded inputs.
+
+Tolerances: complex128 runs match the oracle to relative L2 <= 1e-11 per step
+(rounding of separable kinetic factors and twiddles); complex64 runs to
+relative L2 <= 2e-5 after a few steps.  Shell counts are bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def random_state(n, length, seed=42):
+    x, _ = orc.get_grids(n, length)
+    rng = np.random.default_rng(seed)
+    psi = rng.normal(size=(n,) * 3) + 1j * rng.normal(size=(n,) * 3)
+    return orc.normalise(psi, x)
+
+
+def make(n, length, kind, dt, beta, static=None, dynamic=None, kcut=None, precision="c128"):
+    from gpe_simulator_amd.engine import make_solve
+    problem = orc.Problem(n, length, kind, dt, beta, static=static, dynamic=dynamic, kcut=kcut)
+    solve = make_solve(n, problem.x, problem.k, kind, dt, beta, static=static, dynamic=dynamic, kcut=kcut,
+                       precision=precision)
+    return problem, solve
+
+
+def run_both(problem, solve, psi0, steps, first=0):
+    from gpe_simulator_amd.engine import midpoint_times
+    solve.upload(psi0)
+    tm = midpoint_times(first, steps, problem.dt) if problem.dynamic_source else None
+    solve.step(steps, tm)
+    expected = problem.run(psi0, steps, first=first)
+    return solve.to_numpy(), expected
+
+
+CASES = [
+    # (kind, static, dynamic, kcut)
+    ("real", "x**2 + y**2 + z**2", None, None),
+    ("real", "x/2 + y**2/10 + abs(z)/5", "sin(t)*x + cos(pi)*z", None),
+    ("real", "0.002*(x*x + y*y + z*z)", "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)", 2.5),
+    ("real", "sqrt(x**2 + y**2 + 1) * z / 10", "cos(t)*x*z/20", None),
+    ("imaginary", "x**2 + y**2 + 16*z**2", None, None),
+]
+
+
+@pytest.mark.parametrize("n", [8, 18, 64])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_one_step_c128(n, case, require_gpu):
+    kind, static, dynamic, kcut = CASES[case]
+    length = 8.0
+    problem, solve = make(n, length, kind, 1e-3, 50.0, static, dynamic, kcut)
+    psi0 = random_state(n, length)
+    got, expected = run_both(problem, solve, psi0, 1, first=3)
+    assert rel_l2(got, expected) < 1e-11
+
+
+@pytest.mark.parametrize("n", [64, 96])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_fused_steps_c128(n, case, require_gpu):
+    kind, static, dynamic, kcut = CASES[case]
+    length = 12.0
+    problem, solve = make(n, length, kind, 2e-3, 30.0, static, dynamic, kcut)
+    psi0 = orc.synthetic_state(n, length, seed=1, beta=30.0)
+    got, expected = run_both(problem, solve, psi0, 25)
+    assert rel_l2(got, expected) < 1e-10
+
+
+@pytest.mark.parametrize("n", [128, 192, 256])
+def test_fast_sides_one_step_c128(n, require_gpu):
+    length = 20.0
+    problem, solve = make(n, length, "real", 5e-4, 100.0, "x**2 + y**2 + z**2",
+                          "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)")
+    psi0 = orc.synthetic_state(n, length, seed=2, beta=100.0)
+    got, expected = run_both(problem, solve, psi0, 2)
+    assert rel_l2(got, expected) < 1e-11
+
+
+@pytest.mark.parametrize("n", [18, 64, 96, 256])
+def test_c64_steps(n, require_gpu):
+    length = 16.0
+    problem, solve = make(n, length, "real", 1e-3, 200.0, "x**2 + y**2 + z**2",
+                          "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)", precision="c64")
+    psi0 = orc.synthetic_state(n, length, seed=3, beta=200.0)
+    got, expected = run_both(problem, solve, psi0, 4)
+    assert rel_l2(got, expected) < 2e-5
+
+
+@pytest.mark.parametrize("n", [8, 18, 64, 96])
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_diagnostics(n, precision, require_gpu):
+    length = 10.0
+    problem, solve = make(n, length, "real", 1e-3, 80.0, "x**2 + 0.5*y**2 + z**2",
+                          "cos(t)*y/10", precision=precision)
+    psi0 = random_state(n, length, seed=7)
+    solve.upload(psi0)
+    t = 0.37
+    got = solve.diagnostics(t)
+    exp = problem.energies(psi0, t)
+    tol = 1e-11 if precision == "c128" else 2e-6
+    for key in ("norm", "E_kin", "E_pot", "E_int", "E", "mu", "parseval"):
+        assert got[key] == pytest.approx(exp[key], rel=tol), key
+    spectrum = orc.nk_extraction(np.fft.fftn(psi0), problem.x, problem.k)
+    np.testing.assert_array_equal(got["counts"], spectrum["counts"])
+    dx = problem.x[1] - problem.x[0]
+    np.testing.assert_allclose(got["occupation"] * dx ** 3 / n ** 3, spectrum["Nk"], rtol=tol * 10,
+                               atol=tol * 1e-3)
+
+
+@pytest.mark.parametrize("n", [18, 64])
+def test_imaginary_normalisation_and_history(n, require_gpu):
+    length = 12.0
+    problem, solve = make(n, length, "imaginary", 1e-3, 100.0, "x**2 + y**2 + z**2")
+    psi0 = orc.gaussian(problem.x, 1.0)
+    got, expected = run_both(problem, solve, psi0, 30)
+    assert rel_l2(got, expected) < 1e-10
+    assert solve.norm() == pytest.approx(1.0, abs=1e-12)
+    history = solve.norm_history(30)
+    assert np.all(history < 1.0) and np.all(history > 0.9)
+
+
+def test_gaussian_initial_state(require_gpu):
+    from gpe_simulator_amd.engine import make_solve
+    x, k = orc.get_grids(64, 20.0)
+    solve = make_solve(64, x, k, "imaginary", 1e-3, 0.0)
+    solve.gaussian(1.3)
+    np.testing.assert_allclose(solve.to_numpy(), orc.gaussian(x, 1.3), rtol=1e-13, atol=1e-15)
+    assert solve.norm() == pytest.approx(1.0, abs=1e-13)
