@@ -54,7 +54,7 @@ class SolveDesc(ctypes.Structure):
 
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()
 
 
 def _declare(lib):

@@ -220,6 +220,106 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
   }
 }
 
+// ---------------------------------------------------- global memory access --
+// Pointers read from the device-resident solve descriptor are generic; these
+// casts let the compiler emit global_load/global_store instead of flat ones.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  typedef const __attribute__((address_space(1))) T* gptr;
+  return *(gptr)(p);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  typedef __attribute__((address_space(1))) T* gptr;
+  *(gptr)(p) = v;
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> gload(const cpx<R>* p) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  typedef const __attribute__((address_space(1))) v2* gptr;
+  const v2 v = *(gptr)(p);
+  return cpx<R>{v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> c) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(1))) v2* gptr;
+  v2 v;
+  v.x = c.x;
+  v.y = c.y;
+  *(gptr)(p) = v;
+}
+
+// ----------------------------------------------------------------- sincos --
+// sin and cos of the local-step phase.  Cody-Waite reduction by pi/2 with a
+// three-part split of pi/2 (FMA form), then Taylor polynomials on
+// |r| <= pi/4 whose truncation error is below half an ulp of the format.
+// Arguments beyond the reduction's exact range take the library path (kept
+// out of line so the common path stays small).
+__device__ __attribute__((noinline)) void wide_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __attribute__((noinline)) void wide_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+
+__device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
+  if (!(fabsf(x) < 8192.0f)) {
+    wide_sincos(x, s, c);
+    return;
+  }
+  const float q = rintf(x * 0.636619772367581343f);
+  float r = fmaf(q, -1.5707963705062866f, x);
+  r = fmaf(q, 4.371138828673793e-08f, r);
+  r = fmaf(q, 1.7151245100058819e-15f, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, float(1.0 / 362880.0), float(-1.0 / 5040.0));
+  ps = fmaf(r2, ps, float(1.0 / 120.0));
+  ps = fmaf(r2, ps, float(-1.0 / 6.0));
+  const float sn = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, float(-1.0 / 3628800.0), float(1.0 / 40320.0));
+  pc = fmaf(r2, pc, float(-1.0 / 720.0));
+  pc = fmaf(r2, pc, float(1.0 / 24.0));
+  pc = fmaf(r2, pc, -0.5f);
+  const float cs = fmaf(r2, pc, 1.0f);
+  const int quad = int(q) & 3;
+  const float a = (quad & 1) ? cs : sn;
+  const float b = (quad & 1) ? sn : cs;
+  *s = (quad & 2) ? -a : a;
+  *c = ((quad + 1) & 2) ? -b : b;
+}
+
+__device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
+  if (!(fabs(x) < 1.0e9)) {
+    wide_sincos(x, s, c);
+    return;
+  }
+  const double q = rint(x * 0.63661977236758134308);
+  double r = fma(q, -1.5707963267948966, x);
+  r = fma(q, -6.123233995736766e-17, r);
+  r = fma(q, 1.4973849048591698e-33, r);
+  const double r2 = r * r;
+  double ps = fma(r2, -1.0 / 121645100408832000.0, 1.0 / 355687428096000.0);
+  ps = fma(r2, ps, -1.0 / 1307674368000.0);
+  ps = fma(r2, ps, 1.0 / 6227020800.0);
+  ps = fma(r2, ps, -1.0 / 39916800.0);
+  ps = fma(r2, ps, 1.0 / 362880.0);
+  ps = fma(r2, ps, -1.0 / 5040.0);
+  ps = fma(r2, ps, 1.0 / 120.0);
+  ps = fma(r2, ps, -1.0 / 6.0);
+  const double sn = fma(r * r2, ps, r);
+  double pc = fma(r2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
+  pc = fma(r2, pc, 1.0 / 87178291200.0);
+  pc = fma(r2, pc, -1.0 / 479001600.0);
+  pc = fma(r2, pc, 1.0 / 3628800.0);
+  pc = fma(r2, pc, -1.0 / 40320.0);
+  pc = fma(r2, pc, 1.0 / 720.0);
+  pc = fma(r2, pc, -1.0 / 24.0);
+  pc = fma(r2, pc, 0.5);
+  const double cs = fma(r2, -pc, 1.0);
+  const int quad = int(q) & 3;
+  const double a = (quad & 1) ? cs : sn;
+  const double b = (quad & 1) ? sn : cs;
+  *s = (quad & 2) ? -a : a;
+  *c = ((quad + 1) & 2) ? -b : b;
+}
+
 // ------------------------------------------------------------- reductions --
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -244,19 +244,22 @@ __global__ void k_finish_sums(const double* red, long long nblk, int nq, double*
   }
 }
 
-// Per-bin finish of the shell histogram partials, fixed block order.
+// Per-bin finish of the shell histogram partials: one block per bin, fixed
+// per-thread strides and a fixed tree, so the result is deterministic.
 __global__ void k_finish_nk(const double* occ, const unsigned int* cnt, long long nblk, int nbins, double* out_occ,
                             long long* out_cnt) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nbins) return;
-  double s = 0.0;
-  long long c = 0;
-  for (long long b = 0; b < nblk; ++b) {
-    s += occ[b * nbins + q];
-    c += cnt[b * nbins + q];
+  __shared__ double scratch[16];
+  const int q = blockIdx.x;
+  double acc[2] = {0.0, 0.0};
+  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) {
+    acc[0] += occ[b * nbins + q];
+    acc[1] += double(cnt[b * nbins + q]);
   }
-  out_occ[q] = s;
-  out_cnt[q] = c;
+  block_sum<2>(acc, scratch);
+  if (threadIdx.x == 0) {
+    out_occ[q] = acc[0];
+    out_cnt[q] = (long long)acc[1];
+  }
 }
 
 // Streaming copy used to calibrate bandwidth counters.
@@ -339,8 +342,7 @@ void launch_finish_sums(const double* red, long long nblk, int nq, double* out, 
 }
 void launch_finish_nk(const double* occ, const unsigned int* cnt, long long nblk, int nbins, double* out_occ,
                       long long* out_cnt, hipStream_t st) {
-  hipLaunchKernelGGL(k_finish_nk, dim3(nblocks(nbins, 64)), dim3(64), 0, st, occ, cnt, nblk, nbins, out_occ,
-                     out_cnt);
+  hipLaunchKernelGGL(k_finish_nk, dim3(nbins), dim3(256), 0, st, occ, cnt, nblk, nbins, out_occ, out_cnt);
 }
 void launch_copy(const void* in, void* out, long long bytes, hipStream_t st) {
   const long long count = bytes / 16;

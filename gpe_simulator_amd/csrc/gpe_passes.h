@@ -97,20 +97,28 @@ struct FastTF {
   static constexpr int Q = M / TP;
   static constexpr int THREADS = TP * B;
   static constexpr int LSTRIDE = TP + 1;           // padded LDS row
-  static constexpr int PS = M * LSTRIDE;           // LDS elements per pencil
+  // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
+  // pencils, so an even stride would put them on the same LDS bank.
+  static constexpr int PS = M * LSTRIDE + ((M * LSTRIDE) % 2 == 0 ? 1 : 0);
   static constexpr int LDS_ELEMS = B * PS;
   static_assert(N % TP == 0 && M % TP == 0, "N must be TP*TP*Q");
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + TP * m; }
   __device__ static __forceinline__ bool valid(int, int, int) { return true; }
 
+  // Twiddles W_N^{t*k1} staged once per block in LDS as twl[k1*TP + t].
+  static constexpr int TWL_ELEMS = M * TP;
+  __device__ static __forceinline__ void stage_twiddles(cpx<R>* twl, const cpx<R>* tw) {
+    for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) twl[e] = gload(tw + (e % TP) * (e / TP));
+    __syncthreads();
+  }
+
   template <int DIR>
-  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* __restrict__ tw,
-                                             int) {
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* twl, int) {
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
-      cpx<R> w = tw[t * k1];
+      cpx<R> w = twl[k1 * TP + t];
       if constexpr (DIR > 0) w.y = -w.y;
       v[k1] = cmul(v[k1], w);
     });
@@ -144,6 +152,8 @@ struct GenTF {
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + 64 * m; }
   __device__ static __forceinline__ bool valid(int t, int m, int n) { return t + 64 * m < n; }
+  static constexpr int TWL_ELEMS = 1;
+  __device__ static __forceinline__ void stage_twiddles(cpx<R>*, const cpx<R>*) {}
 
   template <int DIR>
   __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* __restrict__ tw,
@@ -160,7 +170,7 @@ struct GenTF {
         cpx<R> acc{R(0), R(0)};
         int idx = 0;
         for (int s = 0; s < n; ++s) {
-          cpx<R> w = tw[idx];
+          cpx<R> w = gload(tw + idx);
           if constexpr (DIR > 0) w.y = -w.y;
           acc = acc + cmul(lp[s], w);
           idx += k;
@@ -173,16 +183,6 @@ struct GenTF {
 };
 
 // ------------------------------------------------------------- pointwise --
-template <typename R>
-__device__ __forceinline__ void fast_sincos(R x, R* s, R* c);
-template <>
-__device__ __forceinline__ void fast_sincos<float>(float x, float* s, float* c) {
-  sincosf(x, s, c);
-}
-template <>
-__device__ __forceinline__ void fast_sincos<double>(double x, double* s, double* c) {
-  sincos(x, s, c);
-}
 __device__ __forceinline__ float rexp(float x) { return expf(x); }
 __device__ __forceinline__ double rexp(double x) { return exp(x); }
 
@@ -197,12 +197,32 @@ __device__ __forceinline__ cpx<R> half_step(cpx<R> v, R pot, R vim, bool has_vim
     return cpx<R>{v.x * f, v.y * f};
   }
   R sn, cs;
-  fast_sincos<R>(e, &sn, &cs);
+  phase_sincos(e, &sn, &cs);
   cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
   if (has_vim) {
     const R d = rexp(vim * s.hdt);
     r = cpx<R>{r.x * d, r.y * d};
   }
+  return r;
+}
+
+// Two consecutive real-time half-steps (potentials pa then pb) as one phase:
+// the first multiplies |psi| by d = exp(vim dt/2), so the second sees
+// |psi|^2 d^2, and the product of the two factors is d^2 exp(-i(ea + eb)).
+template <typename R>
+__device__ __forceinline__ cpx<R> two_half_steps(cpx<R> v, R pa, R pb, R vim, bool has_vim,
+                                                 const SolveDev<R>& s) {
+  const R a2 = v.x * v.x + v.y * v.y;
+  R d2 = R(1);
+  if (has_vim) {
+    const R d = rexp(vim * s.hdt);
+    d2 = d * d;
+  }
+  const R e = (pa + s.beta * a2) * s.hdt + (pb + s.beta * (a2 * d2)) * s.hdt;
+  R sn, cs;
+  phase_sincos(e, &sn, &cs);
+  cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+  if (has_vim) r = cpx<R>{r.x * d2, r.y * d2};
   return r;
 }
 
@@ -254,44 +274,47 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
     stride = (long long)n * n;
   }
   cpx<R>* lp = lds + b * TF::PS;
+  __shared__ cpx<R> twl_lds[TF::TWL_ELEMS];
+  TF::stage_twiddles(twl_lds, s.tw);
+  const cpx<R>* twp = TF::kFast ? static_cast<const cpx<R>*>(twl_lds) : s.tw;
   const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
 
   cpx<R> v[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    if (TF::valid(t, m, n)) v[m] = src[base + (long long)TF::pos(t, m) * stride];
+    if (TF::valid(t, m, n)) v[m] = gload(src + base + (long long)TF::pos(t, m) * stride);
   }
 
   if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD) {
-    TF::template fft<-1>(v, lp, t, s.tw, n);
+    TF::template fft<-1>(v, lp, t, twp, n);
   } else if constexpr (KIND == K_Y_INV) {
-    TF::template fft<+1>(v, lp, t, s.tw, n);
+    TF::template fft<+1>(v, lp, t, twp, n);
   } else if constexpr (KIND == K_X_KIN) {
-    TF::template fft<-1>(v, lp, t, s.tw, n);
+    TF::template fft<-1>(v, lp, t, twp, n);
     if (s.imag) {
-      const R eyz = s.kin_y[pj].x * s.kin_z[pk].x;
+      const R eyz = gload(s.kin_y + (pj)).x * gload(s.kin_z + (pk)).x;
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (TF::valid(t, m, n)) {
-          const R f = s.kin_x[TF::pos(t, m)].x * eyz;
+          const R f = gload(s.kin_x + (TF::pos(t, m))).x * eyz;
           v[m] = cpx<R>{v[m].x * f, v[m].y * f};
         }
       }
     } else {
-      const cpx<R> eyz = cmul(s.kin_y[pj], s.kin_z[pk]);
+      const cpx<R> eyz = cmul(gload(s.kin_y + (pj)), gload(s.kin_z + (pk)));
       double ky2 = 0.0, kz2 = 0.0;
       if (s.masked) {
-        ky2 = __dmul_rn(s.karr[pj], s.karr[pj]);
-        kz2 = __dmul_rn(s.karr[pk], s.karr[pk]);
+        ky2 = __dmul_rn(gload(s.karr + (pj)), gload(s.karr + (pj)));
+        kz2 = __dmul_rn(gload(s.karr + (pk)), gload(s.karr + (pk)));
       }
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (TF::valid(t, m, n)) {
           const int i = TF::pos(t, m);
-          cpx<R> f = cmul(s.kin_x[i], eyz);
+          cpx<R> f = cmul(gload(s.kin_x + (i)), eyz);
           if (s.masked) {
-            const double kx = s.karr[i];
+            const double kx = gload(s.karr + (i));
             // same association as kx*kx + ky*ky + kz*kz, without contraction
             const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
             if (k2 > s.kcut2) f = cpx<R>{R(0), R(0)};
@@ -300,10 +323,10 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
         }
       }
     }
-    TF::template fft<+1>(v, lp, t, s.tw, n);
+    TF::template fft<+1>(v, lp, t, twp, n);
   } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST) {
     if constexpr (KIND != K_Z_FIRST) {
-      TF::template fft<+1>(v, lp, t, s.tw, n);
+      TF::template fft<+1>(v, lp, t, twp, n);
     } else {
       if (a.flags & F_PRESCALE) {
         const R sc = R(s.scal[1]);
@@ -312,9 +335,9 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       }
     }
     const long long rowoff = ((long long)pi * n + pj) * n;
-    const R pxa = s.pxy[a.slot_a * s.pxy_ss + (long long)pi * n + pj];
+    const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
     R pxb = R(0);
-    if constexpr (KIND == K_Z_MID) pxb = s.pxy[a.slot_b * s.pxy_ss + (long long)pi * n + pj];
+    if constexpr (KIND == K_Z_MID) pxb = gload(s.pxy + (a.slot_b * s.pxy_ss + (long long)pi * n + pj));
     const bool has_vim = s.vim != nullptr;
     double nsum = 0.0;
 #pragma unroll
@@ -322,15 +345,19 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       if (TF::valid(t, m, n)) {
         const int k = TF::pos(t, m);
         R vst = R(0), vi = R(0);
-        if (s.vre) vst = s.vre[rowoff + k];
-        if (has_vim) vi = s.vim[rowoff + k];
-        R pa = pxa + s.pz[a.slot_a * s.pz_ss + k] + vst;
-        if (s.vdyn) pa += s.vdyn[a.slot_a * s.vdyn_ss + rowoff + k];
-        cpx<R> w = half_step(v[m], pa, vi, has_vim, s);
+        if (s.vre) vst = gload(s.vre + (rowoff + k));
+        if (has_vim) vi = gload(s.vim + (rowoff + k));
+        const R pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
+        R pa = pxa + pza + vst;
+        if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
+        cpx<R> w;
         if constexpr (KIND == K_Z_MID) {
-          R pb = pxb + s.pz[a.slot_b * s.pz_ss + k] + vst;
-          if (s.vdyn) pb += s.vdyn[a.slot_b * s.vdyn_ss + rowoff + k];
-          w = half_step(w, pb, vi, has_vim, s);
+          const R pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
+          R pb = pxb + pzb + vst;
+          if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
+          w = two_half_steps(v[m], pa, pb, vi, has_vim, s);
+        } else {
+          w = half_step(v[m], pa, vi, has_vim, s);
         }
         if constexpr (KIND == K_Z_LAST) {
           nsum += double(w.x) * double(w.x) + double(w.y) * double(w.y);
@@ -339,7 +366,7 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       }
     }
     if constexpr (KIND != K_Z_LAST) {
-      TF::template fft<-1>(v, lp, t, s.tw, n);
+      TF::template fft<-1>(v, lp, t, twp, n);
     } else {
       if (a.flags & F_NORM) {
         // FP64 block partial, then the last block to finish sums all
@@ -380,15 +407,15 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
     }
     const long long rowoff = ((long long)pi * n + pj) * n;
-    const double pxa = double(s.pxy[a.slot_a * s.pxy_ss + (long long)pi * n + pj]);
+    const double pxa = double(gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj)));
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       if (TF::valid(t, m, n)) {
         const int k = TF::pos(t, m);
-        double pot = pxa + double(s.pz[a.slot_a * s.pz_ss + k]);
-        if (s.vre) pot += double(s.vre[rowoff + k]);
-        if (s.vdyn) pot += double(s.vdyn[a.slot_a * s.vdyn_ss + rowoff + k]);
+        double pot = pxa + double(gload(s.pz + (a.slot_a * s.pz_ss + k)));
+        if (s.vre) pot += double(gload(s.vre + (rowoff + k)));
+        if (s.vdyn) pot += double(gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k)));
         const double a2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
         acc[0] += a2;
         acc[1] += pot * a2;
@@ -403,7 +430,7 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       s.red[bid * 3 + 1] = acc[1];
       s.red[bid * 3 + 2] = acc[2];
     }
-    TF::template fft<-1>(v, lp, t, s.tw, n);
+    TF::template fft<-1>(v, lp, t, twp, n);
   } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
     // Angular momentum by Parseval along one axis:
     //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
@@ -412,14 +439,14 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
 #pragma unroll
       for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
     }
-    TF::template fft<-1>(v, lp, t, s.tw, n);
-    const double coord = KIND == K_Y_LZ ? s.xarr[pi] : s.xarr[pj];
+    TF::template fft<-1>(v, lp, t, twp, n);
+    const double coord = KIND == K_Y_LZ ? gload(s.xarr + (pi)) : gload(s.xarr + (pj));
     double acc[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       if (TF::valid(t, m, n)) {
         const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
-        acc[0] += s.karr[TF::pos(t, m)] * f2;
+        acc[0] += gload(s.karr + (TF::pos(t, m))) * f2;
       }
     }
     acc[0] *= coord;
@@ -428,7 +455,7 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
     if (threadIdx.x == 0) s.red[blockIdx.x + gridDim.x * blockIdx.y] = acc[0];
     return;
   } else if constexpr (KIND == K_X_DIAG) {
-    TF::template fft<-1>(v, lp, t, s.tw, n);
+    TF::template fft<-1>(v, lp, t, twp, n);
     __shared__ double hocc[1024];
     __shared__ unsigned int hcnt[1024];
     for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
@@ -436,14 +463,14 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
       hcnt[q] = 0u;
     }
     __syncthreads();
-    const double ky = s.karr[pj], kz = s.karr[pk];
+    const double ky = gload(s.karr + (pj)), kz = gload(s.karr + (pk));
     const int fq = freq_index(pj, n), fr = freq_index(pk, n);
     double acc[2] = {0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       if (TF::valid(t, m, n)) {
         const int i = TF::pos(t, m);
-        const double kx = s.karr[i];
+        const double kx = gload(s.karr + (i));
         const double k2 = kx * kx + ky * ky + kz * kz;
         const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
         acc[0] += f2 * k2;
@@ -470,7 +497,7 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
 
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    if (TF::valid(t, m, n)) dst[base + (long long)TF::pos(t, m) * stride] = v[m];
+    if (TF::valid(t, m, n)) gstore(dst + base + (long long)TF::pos(t, m) * stride, v[m]);
   }
 }
 
