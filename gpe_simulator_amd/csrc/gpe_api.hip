@@ -91,7 +91,8 @@ struct gpe_solve {
   size_t esz = 0, rsz = 0;
   hipStream_t st = nullptr;
   PassLauncher launch = nullptr;
-  PassBlocks blocks = nullptr;
+  PassBlocks blocks_fn = nullptr;
+  long long blocks(int kind, int n) const { return blocks_fn(kind, n); }
 
   void* psi = nullptr;
   void* work = nullptr;
@@ -581,7 +582,7 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   s->nbins = nbins_for(d->n);
   const double dx = d->x_array[1] - d->x_array[0];
   s->dx3 = dx * dx * dx;
-  pick_launchers(s->prec, s->n, &s->launch, &s->blocks);
+  pick_launchers(s->prec, s->n, &s->launch, &s->blocks_fn);
 
   auto bail = [&](int code) {
     gpe_solve_destroy(s);
@@ -900,12 +901,12 @@ int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, do
 int gpe_angular_momentum(gpe_solve* s, double* out) {
   if (!s || !out) return fail(GPE_ERR_INVALID, "null argument");
   CHK(set_device(s));
-  const long long bx = s->blocks(K_X_LZ, s->n);
-  CHK(ensure_red(s, bx));
+  const long long bx = s->blocks(K_X_LZ, s->n), by = s->blocks(K_Y_LZ, s->n);
+  CHK(ensure_red(s, std::max(bx, by)));
   if (!s->d_out) CHK(dzalloc(&s->d_out, 16 * sizeof(double)));
   const int pre = s->pending ? F_PRESCALE : 0;
   CHK(pass(s, K_Y_LZ, true, 0, 0, pre));
-  launch_finish_sums(s->red, bx, 1, s->d_out + 8, s->st);
+  launch_finish_sums(s->red, by, 1, s->d_out + 8, s->st);
   CHK(pass(s, K_X_LZ, true, 0, 0, pre));
   launch_finish_sums(s->red, bx, 1, s->d_out + 9, s->st);
   HIPCHK(hipGetLastError());

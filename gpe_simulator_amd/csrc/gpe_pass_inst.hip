@@ -31,37 +31,44 @@ using TF = typename Shape<Real, GPE_N>::TF;
 #define GPE_LAUNCHER GPE_CAT(GPE_CAT(GPE_CAT(launch_pass_p, GPE_PREC), _n), GPE_N)
 
 template <int AXIS, int KIND>
-static void go(dim3 grid, hipStream_t st, const PassArgs<Real>& a) {
+static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a) {
+  const dim3 grid((unsigned)tile_count<TF, AXIS>(n), 1, (unsigned)batch);
   hipLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, a);
 }
 
-// Launches one pass over `batch` solves of side n sharing one descriptor array.
+static int axis_of(int kind) {
+  switch (kind) {
+    case K_Y_FWD: case K_Y_INV: case K_Y_LZ: return 1;
+    case K_X_KIN: case K_X_DIAG: case K_X_LZ: return 0;
+    default: return 2;
+  }
+}
+
+// Launches one pass over `batch` solves of side n sharing one descriptor
+// array; one block per tile of B pencils.
 hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_t st) {
   const PassArgs<Real>& a = *static_cast<const PassArgs<Real>*>(args);
-  const dim3 gz((unsigned)((long long)n * n / TF::B), 1, (unsigned)batch);
-  const dim3 gxy((unsigned)(n / TF::B), (unsigned)n, (unsigned)batch);
   switch (kind) {
-    case K_Z_FIRST: go<2, K_Z_FIRST>(gz, st, a); break;
-    case K_Z_MID: go<2, K_Z_MID>(gz, st, a); break;
-    case K_Z_LAST: go<2, K_Z_LAST>(gz, st, a); break;
-    case K_Z_DIAG: go<2, K_Z_DIAG>(gz, st, a); break;
-    case K_Z_FWD: go<2, K_Z_FWD>(gz, st, a); break;
-    case K_Y_FWD: go<1, K_Y_FWD>(gxy, st, a); break;
-    case K_Y_INV: go<1, K_Y_INV>(gxy, st, a); break;
-    case K_X_KIN: go<0, K_X_KIN>(gxy, st, a); break;
-    case K_X_DIAG: go<0, K_X_DIAG>(gxy, st, a); break;
-    case K_Y_LZ: go<1, K_Y_LZ>(gxy, st, a); break;
-    case K_X_LZ: go<0, K_X_LZ>(gxy, st, a); break;
+    case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a); break;
+    case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a); break;
+    case K_Z_LAST: go<2, K_Z_LAST>(n, batch, st, a); break;
+    case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a); break;
+    case K_Z_FWD: go<2, K_Z_FWD>(n, batch, st, a); break;
+    case K_Y_FWD: go<1, K_Y_FWD>(n, batch, st, a); break;
+    case K_Y_INV: go<1, K_Y_INV>(n, batch, st, a); break;
+    case K_X_KIN: go<0, K_X_KIN>(n, batch, st, a); break;
+    case K_X_DIAG: go<0, K_X_DIAG>(n, batch, st, a); break;
+    case K_Y_LZ: go<1, K_Y_LZ>(n, batch, st, a); break;
+    case K_X_LZ: go<0, K_X_LZ>(n, batch, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// Number of blocks one pass of `kind` uses per solve (for partial buffers).
+// Blocks per solve that a pass of `kind` launches (sizes partial buffers).
 long long GPE_CAT(GPE_CAT(GPE_CAT(pass_blocks_p, GPE_PREC), _n), GPE_N)(int kind, int n) {
-  if (kind == K_Z_FIRST || kind == K_Z_MID || kind == K_Z_LAST || kind == K_Z_DIAG || kind == K_Z_FWD)
-    return (long long)n * n / TF::B;
-  return (long long)(n / TF::B) * n;
+  const int axis = axis_of(kind);
+  return axis == 2 ? tile_count<TF, 2>(n) : (axis == 1 ? tile_count<TF, 1>(n) : tile_count<TF, 0>(n));
 }
 
 }  // namespace gpe

@@ -96,6 +96,9 @@ struct FastTF {
   static constexpr int M = N / TP;
   static constexpr int Q = M / TP;
   static constexpr int THREADS = TP * B;
+  // Waves per SIMD the register allocation must allow (LDS caps residency at
+  // about four 256-thread blocks per CU, so more registers buy nothing).
+  static constexpr int MIN_WAVES = sizeof(R) == 4 ? 4 : 2;
   static constexpr int LSTRIDE = TP + 1;           // padded LDS row
   // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
   // pencils, so an even stride would put them on the same LDS bank.
@@ -147,6 +150,7 @@ struct GenTF {
   static constexpr bool kFast = false;
   static constexpr int N = 0, TP = 64, B = 1, M = 16;
   static constexpr int THREADS = 64;
+  static constexpr int MIN_WAVES = 1;
   static constexpr int PS = 1024;
   static constexpr int LDS_ELEMS = 1024;
 
@@ -239,9 +243,51 @@ __device__ __forceinline__ int shell_index(int p, int q, int r) {
 }
 
 // ------------------------------------------------------------ pass kernel --
+// Tile = B pencils; one block per tile.  (A multi-tile variant that prefetched
+// the next tile into registers measured slower at every size: the live
+// prefetch registers cost more occupancy than the overlap gained.)
+template <class TF, int AXIS>
+struct Tile {
+  long long base;
+  int pi, pj, pk;  // pencil coordinates (the axis coordinate is unused)
+  __device__ __forceinline__ Tile(int tile, int b, int n) : pi(0), pj(0), pk(0) {
+    if constexpr (AXIS == 2) {
+      const long long row = (long long)tile * TF::B + b;
+      pi = int(row / n);
+      pj = int(row % n);
+      base = row * n;
+    } else {
+      const int tiles_per_row = n / TF::B;
+      const int outer = tile / tiles_per_row;
+      pk = (tile - outer * tiles_per_row) * TF::B + b;
+      if constexpr (AXIS == 1) {
+        pi = outer;
+        base = (long long)pi * n * n + pk;
+      } else {
+        pj = outer;
+        base = (long long)pj * n + pk;
+      }
+    }
+  }
+};
+
+template <class TF, int AXIS>
+__host__ __device__ __forceinline__ int tile_count(int n) {
+  return AXIS == 2 ? int((long long)n * n / TF::B) : (n / TF::B) * n;
+}
+
+template <class TF, typename R>
+__device__ __forceinline__ void load_tile(cpx<R>* v, const cpx<R>* src, long long base, long long stride, int t,
+                                          int n) {
+#pragma unroll
+  for (int m = 0; m < TF::M; ++m)
+    if (TF::valid(t, m, n)) v[m] = gload(src + base + (long long)TF::pos(t, m) * stride);
+}
+
 template <class TF, int AXIS, int KIND, typename R>
-__global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
+__global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R> a) {
   __shared__ cpx<R> lds[TF::LDS_ELEMS];
+  __shared__ cpx<R> twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
   constexpr int M = TF::M;
@@ -254,250 +300,241 @@ __global__ __launch_bounds__(TF::THREADS) void k_pass(PassArgs<R> a) {
     b = threadIdx.x % TF::B;
     t = threadIdx.x / TF::B;
   }
-  long long base, stride;
-  int pi = 0, pj = 0, pk = 0;  // pencil coordinates (axis coordinate unused)
-  if constexpr (AXIS == 2) {
-    const long long row = (long long)blockIdx.x * TF::B + b;
-    pi = int(row / n);
-    pj = int(row % n);
-    base = row * n;
-    stride = 1;
-  } else if constexpr (AXIS == 1) {
-    pi = blockIdx.y;
-    pk = blockIdx.x * TF::B + b;
-    base = (long long)pi * n * n + pk;
-    stride = n;
-  } else {
-    pj = blockIdx.y;
-    pk = blockIdx.x * TF::B + b;
-    base = (long long)pj * n + pk;
-    stride = (long long)n * n;
-  }
+  const long long stride = AXIS == 2 ? 1LL : (AXIS == 1 ? (long long)n : (long long)n * n);
+  const int ntiles = tile_count<TF, AXIS>(n);
   cpx<R>* lp = lds + b * TF::PS;
-  __shared__ cpx<R> twl_lds[TF::TWL_ELEMS];
   TF::stage_twiddles(twl_lds, s.tw);
   const cpx<R>* twp = TF::kFast ? static_cast<const cpx<R>*>(twl_lds) : s.tw;
   const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
+  const bool has_vim = s.vim != nullptr;
+  const bool prescale = (a.flags & F_PRESCALE) != 0;
+  const R sc = prescale ? R(s.scal[1]) : R(1);
 
-  cpx<R> v[M];
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    if (TF::valid(t, m, n)) v[m] = gload(src + base + (long long)TF::pos(t, m) * stride);
-  }
-
-  if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD) {
-    TF::template fft<-1>(v, lp, t, twp, n);
-  } else if constexpr (KIND == K_Y_INV) {
-    TF::template fft<+1>(v, lp, t, twp, n);
-  } else if constexpr (KIND == K_X_KIN) {
-    TF::template fft<-1>(v, lp, t, twp, n);
-    if (s.imag) {
-      const R eyz = gload(s.kin_y + (pj)).x * gload(s.kin_z + (pk)).x;
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        if (TF::valid(t, m, n)) {
-          const R f = gload(s.kin_x + (TF::pos(t, m))).x * eyz;
-          v[m] = cpx<R>{v[m].x * f, v[m].y * f};
-        }
-      }
-    } else {
-      const cpx<R> eyz = cmul(gload(s.kin_y + (pj)), gload(s.kin_z + (pk)));
-      double ky2 = 0.0, kz2 = 0.0;
-      if (s.masked) {
-        ky2 = __dmul_rn(gload(s.karr + (pj)), gload(s.karr + (pj)));
-        kz2 = __dmul_rn(gload(s.karr + (pk)), gload(s.karr + (pk)));
-      }
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        if (TF::valid(t, m, n)) {
-          const int i = TF::pos(t, m);
-          cpx<R> f = cmul(gload(s.kin_x + (i)), eyz);
-          if (s.masked) {
-            const double kx = gload(s.karr + (i));
-            // same association as kx*kx + ky*ky + kz*kz, without contraction
-            const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
-            if (k2 > s.kcut2) f = cpx<R>{R(0), R(0)};
-          }
-          v[m] = cmul(v[m], f);
-        }
-      }
-    }
-    TF::template fft<+1>(v, lp, t, twp, n);
-  } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST) {
-    if constexpr (KIND != K_Z_FIRST) {
-      TF::template fft<+1>(v, lp, t, twp, n);
-    } else {
-      if (a.flags & F_PRESCALE) {
-        const R sc = R(s.scal[1]);
-#pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
-      }
-    }
-    const long long rowoff = ((long long)pi * n + pj) * n;
-    const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
-    R pxb = R(0);
-    if constexpr (KIND == K_Z_MID) pxb = gload(s.pxy + (a.slot_b * s.pxy_ss + (long long)pi * n + pj));
-    const bool has_vim = s.vim != nullptr;
-    double nsum = 0.0;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (TF::valid(t, m, n)) {
-        const int k = TF::pos(t, m);
-        R vst = R(0), vi = R(0);
-        if (s.vre) vst = gload(s.vre + (rowoff + k));
-        if (has_vim) vi = gload(s.vim + (rowoff + k));
-        const R pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
-        R pa = pxa + pza + vst;
-        if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
-        cpx<R> w;
-        if constexpr (KIND == K_Z_MID) {
-          const R pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
-          R pb = pxb + pzb + vst;
-          if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
-          w = two_half_steps(v[m], pa, pb, vi, has_vim, s);
-        } else {
-          w = half_step(v[m], pa, vi, has_vim, s);
-        }
-        if constexpr (KIND == K_Z_LAST) {
-          nsum += double(w.x) * double(w.x) + double(w.y) * double(w.y);
-        }
-        v[m] = w;
-      }
-    }
-    if constexpr (KIND != K_Z_LAST) {
-      TF::template fft<-1>(v, lp, t, twp, n);
-    } else {
-      if (a.flags & F_NORM) {
-        // FP64 block partial, then the last block to finish sums all
-        // partials in a fixed order and publishes norm and 1/sqrt(norm).
-        __shared__ double red_scratch[16];
-        __shared__ int is_last;
-        double acc[1] = {nsum};
-        block_sum<1>(acc, red_scratch);
-        const unsigned nblk = gridDim.x * gridDim.y;
-        const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
-        if (threadIdx.x == 0) {
-          s.red[bid] = acc[0];
-          __threadfence();
-          const unsigned ticket = atomicAdd(s.counter, 1u);
-          is_last = (ticket == nblk - 1);
-        }
-        __syncthreads();
-        if (is_last) {
-          __threadfence();
-          double part[1] = {0.0};
-          for (unsigned q = threadIdx.x; q < nblk; q += blockDim.x) part[0] += coherent_load(s.red + q);
-          block_sum<1>(part, red_scratch);
-          if (threadIdx.x == 0) {
-            const double norm = s.dx3 * part[0];
-            s.scal[0] = norm;
-            s.scal[1] = 1.0 / sqrt(norm);
-            if (s.hist) s.hist[a.hist_index] = norm;
-            *s.counter = 0u;
-          }
-        }
-      }
-    }
-  } else if constexpr (KIND == K_Z_DIAG) {
-    // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
-    if (a.flags & F_PRESCALE) {
-      const R sc = R(s.scal[1]);
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
-    }
-    const long long rowoff = ((long long)pi * n + pj) * n;
-    const double pxa = double(gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj)));
-    double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (TF::valid(t, m, n)) {
-        const int k = TF::pos(t, m);
-        double pot = pxa + double(gload(s.pz + (a.slot_a * s.pz_ss + k)));
-        if (s.vre) pot += double(gload(s.vre + (rowoff + k)));
-        if (s.vdyn) pot += double(gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k)));
-        const double a2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
-        acc[0] += a2;
-        acc[1] += pot * a2;
-        acc[2] += a2 * a2;
-      }
-    }
-    __shared__ double red_scratch[48];
-    block_sum<3>(acc, red_scratch);
-    if (threadIdx.x == 0) {
-      const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
-      s.red[bid * 3 + 0] = acc[0];
-      s.red[bid * 3 + 1] = acc[1];
-      s.red[bid * 3 + 2] = acc[2];
-    }
-    TF::template fft<-1>(v, lp, t, twp, n);
-  } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
-    // Angular momentum by Parseval along one axis:
-    //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
-    if (a.flags & F_PRESCALE) {
-      const R sc = R(s.scal[1]);
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
-    }
-    TF::template fft<-1>(v, lp, t, twp, n);
-    const double coord = KIND == K_Y_LZ ? gload(s.xarr + (pi)) : gload(s.xarr + (pj));
-    double acc[1] = {0.0};
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (TF::valid(t, m, n)) {
-        const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
-        acc[0] += gload(s.karr + (TF::pos(t, m))) * f2;
-      }
-    }
-    acc[0] *= coord;
-    __shared__ double red_scratch[16];
-    block_sum<1>(acc, red_scratch);
-    if (threadIdx.x == 0) s.red[blockIdx.x + gridDim.x * blockIdx.y] = acc[0];
-    return;
-  } else if constexpr (KIND == K_X_DIAG) {
-    TF::template fft<-1>(v, lp, t, twp, n);
-    __shared__ double hocc[1024];
-    __shared__ unsigned int hcnt[1024];
+  // Reductions accumulated over all tiles of the block.
+  double acc[3] = {0.0, 0.0, 0.0};
+  __shared__ double hocc[KIND == K_X_DIAG ? 1024 : 1];
+  __shared__ unsigned int hcnt[KIND == K_X_DIAG ? 1024 : 1];
+  if constexpr (KIND == K_X_DIAG) {
     for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
       hocc[q] = 0.0;
       hcnt[q] = 0u;
     }
-    __syncthreads();
-    const double ky = gload(s.karr + (pj)), kz = gload(s.karr + (pk));
-    const int fq = freq_index(pj, n), fr = freq_index(pk, n);
-    double acc[2] = {0.0, 0.0};
+  }
+
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  cpx<R> v[M];
+  {
+    const Tile<TF, AXIS> g(tile, b, n);
+    load_tile<TF>(v, src, g.base, stride, t, n);
+  }
+  {
+    const Tile<TF, AXIS> g(tile, b, n);
+    const int pi = g.pi, pj = g.pj, pk = g.pk;
+    bool write = true;
+
+    if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD) {
+      TF::template fft<-1>(v, lp, t, twp, n);
+    } else if constexpr (KIND == K_Y_INV) {
+      TF::template fft<+1>(v, lp, t, twp, n);
+    } else if constexpr (KIND == K_X_KIN) {
+      TF::template fft<-1>(v, lp, t, twp, n);
+      if (s.imag) {
+        const R eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (TF::valid(t, m, n)) {
-        const int i = TF::pos(t, m);
-        const double kx = gload(s.karr + (i));
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
-        acc[0] += f2 * k2;
-        acc[1] += f2;
-        const int sh = shell_index(freq_index(i, n), fq, fr);
-        atomicAdd(&hocc[sh], f2);
-        atomicAdd(&hcnt[sh], 1u);
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) {
+            const R f = gload(s.kin_x + TF::pos(t, m)).x * eyz;
+            v[m] = cpx<R>{v[m].x * f, v[m].y * f};
+          }
+        }
+      } else {
+        const cpx<R> eyz = cmul(gload(s.kin_y + pj), gload(s.kin_z + pk));
+        double ky2 = 0.0, kz2 = 0.0;
+        if (s.masked) {
+          ky2 = __dmul_rn(gload(s.karr + pj), gload(s.karr + pj));
+          kz2 = __dmul_rn(gload(s.karr + pk), gload(s.karr + pk));
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) {
+            const int i = TF::pos(t, m);
+            cpx<R> f = cmul(gload(s.kin_x + i), eyz);
+            if (s.masked) {
+              const double kx = gload(s.karr + i);
+              // same association as kx*kx + ky*ky + kz*kz, without contraction
+              const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
+              if (k2 > s.kcut2) f = cpx<R>{R(0), R(0)};
+            }
+            v[m] = cmul(v[m], f);
+          }
+        }
+      }
+      TF::template fft<+1>(v, lp, t, twp, n);
+    } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST) {
+      if constexpr (KIND != K_Z_FIRST) {
+        TF::template fft<+1>(v, lp, t, twp, n);
+      } else if (prescale) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+      }
+      const long long rowoff = g.base;
+      const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
+      R pxb = R(0);
+      if constexpr (KIND == K_Z_MID) pxb = gload(s.pxy + (a.slot_b * s.pxy_ss + (long long)pi * n + pj));
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const int k = TF::pos(t, m);
+          R vst = R(0), vi = R(0);
+          if (s.vre) vst = gload(s.vre + (rowoff + k));
+          if (has_vim) vi = gload(s.vim + (rowoff + k));
+          const R pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
+          R pa = pxa + pza + vst;
+          if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
+          cpx<R> w;
+          if constexpr (KIND == K_Z_MID) {
+            const R pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
+            R pb = pxb + pzb + vst;
+            if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
+            w = two_half_steps(v[m], pa, pb, vi, has_vim, s);
+          } else {
+            w = half_step(v[m], pa, vi, has_vim, s);
+          }
+          if constexpr (KIND == K_Z_LAST) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+          v[m] = w;
+        }
+      }
+      if constexpr (KIND != K_Z_LAST) TF::template fft<-1>(v, lp, t, twp, n);
+    } else if constexpr (KIND == K_Z_DIAG) {
+      // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
+      if (prescale) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+      }
+      const long long rowoff = g.base;
+      const double pxa = double(gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj)));
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const int k = TF::pos(t, m);
+          double pot = pxa + double(gload(s.pz + (a.slot_a * s.pz_ss + k)));
+          if (s.vre) pot += double(gload(s.vre + (rowoff + k)));
+          if (s.vdyn) pot += double(gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k)));
+          const double a2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+          acc[0] += a2;
+          acc[1] += pot * a2;
+          acc[2] += a2 * a2;
+        }
+      }
+      TF::template fft<-1>(v, lp, t, twp, n);
+    } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
+      // Angular momentum by Parseval along one axis:
+      //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
+      if (prescale) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+      }
+      TF::template fft<-1>(v, lp, t, twp, n);
+      const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);
+      double part = 0.0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+          part += gload(s.karr + TF::pos(t, m)) * f2;
+        }
+      }
+      acc[0] += coord * part;
+      write = false;
+    } else if constexpr (KIND == K_X_DIAG) {
+      TF::template fft<-1>(v, lp, t, twp, n);
+      const double ky = gload(s.karr + pj), kz = gload(s.karr + pk);
+      const int fq = freq_index(pj, n), fr = freq_index(pk, n);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (TF::valid(t, m, n)) {
+          const int i = TF::pos(t, m);
+          const double kx = gload(s.karr + i);
+          const double k2 = kx * kx + ky * ky + kz * kz;
+          const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+          acc[0] += f2 * k2;
+          acc[1] += f2;
+          const int sh = shell_index(freq_index(i, n), fq, fr);
+          atomicAdd(&hocc[sh], f2);
+          atomicAdd(&hcnt[sh], 1u);
+        }
+      }
+      write = false;
+    }
+
+    if (write) {
+      const long long base = g.base;
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+        if (TF::valid(t, m, n)) gstore(dst + base + (long long)TF::pos(t, m) * stride, v[m]);
+    }
+  }
+
+  // ---- per-block reductions
+  const unsigned nblk = gridDim.x;
+  const unsigned bid = blockIdx.x;
+  if constexpr (KIND == K_Z_LAST) {
+    if (a.flags & F_NORM) {
+      // FP64 block partial, then the last block to finish sums all partials
+      // in a fixed order and publishes norm and 1/sqrt(norm).
+      __shared__ double red_scratch[16];
+      __shared__ int is_last;
+      double part[1] = {acc[0]};
+      block_sum<1>(part, red_scratch);
+      if (threadIdx.x == 0) {
+        s.red[bid] = part[0];
+        __threadfence();
+        const unsigned ticket = atomicAdd(s.counter, 1u);
+        is_last = (ticket == nblk - 1);
+      }
+      __syncthreads();
+      if (is_last) {
+        __threadfence();
+        double tot[1] = {0.0};
+        for (unsigned q = threadIdx.x; q < nblk; q += blockDim.x) tot[0] += coherent_load(s.red + q);
+        block_sum<1>(tot, red_scratch);
+        if (threadIdx.x == 0) {
+          const double norm = s.dx3 * tot[0];
+          s.scal[0] = norm;
+          s.scal[1] = 1.0 / sqrt(norm);
+          if (s.hist) s.hist[a.hist_index] = norm;
+          *s.counter = 0u;
+        }
       }
     }
-    __shared__ double red_scratch[32];
-    block_sum<2>(acc, red_scratch);
-    const unsigned bid = blockIdx.x + gridDim.x * blockIdx.y;
+  } else if constexpr (KIND == K_Z_DIAG) {
+    __shared__ double red_scratch[48];
+    block_sum<3>(acc, red_scratch);
     if (threadIdx.x == 0) {
-      s.red[bid * 2 + 0] = acc[0];
-      s.red[bid * 2 + 1] = acc[1];
+      s.red[bid * 3 + 0] = acc[0];
+      s.red[bid * 3 + 1] = acc[1];
+      s.red[bid * 3 + 2] = acc[2];
+    }
+  } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
+    __shared__ double red_scratch[16];
+    double part[1] = {acc[0]};
+    block_sum<1>(part, red_scratch);
+    if (threadIdx.x == 0) s.red[bid] = part[0];
+  } else if constexpr (KIND == K_X_DIAG) {
+    __shared__ double red_scratch[32];
+    double part[2] = {acc[0], acc[1]};
+    block_sum<2>(part, red_scratch);
+    if (threadIdx.x == 0) {
+      s.red[bid * 2 + 0] = part[0];
+      s.red[bid * 2 + 1] = part[1];
     }
     __syncthreads();
     for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
       s.nkocc[(long long)bid * s.nbins + q] = hocc[q];
       s.nkcnt[(long long)bid * s.nbins + q] = hcnt[q];
     }
-    return;
-  }
-
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    if (TF::valid(t, m, n)) gstore(dst + base + (long long)TF::pos(t, m) * stride, v[m]);
   }
 }
 
