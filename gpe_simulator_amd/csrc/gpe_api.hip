@@ -177,10 +177,10 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
   const long long nn = (long long)s->n * s->n;
   d.psi = (cpx<R>*)s->psi;
   d.work = (cpx<R>*)s->work;
-  d.tw = (const cpx<R>*)s->tw;
-  d.kin_x = (const cpx<R>*)s->kin[0];
-  d.kin_y = (const cpx<R>*)s->kin[1];
-  d.kin_z = (const cpx<R>*)s->kin[2];
+  d.tw = (const cpx<double>*)s->tw;
+  d.kin_x = (const cpx<double>*)s->kin[0];
+  d.kin_y = (const cpx<double>*)s->kin[1];
+  d.kin_z = (const cpx<double>*)s->kin[2];
   d.karr = s->ka;
   d.xarr = s->xa;
   if (s->dyn[0].active) {
@@ -595,9 +595,9 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   }
   int r;
   const int n = s->n;
-  if ((r = dzalloc(&s->psi, s->n3 * s->esz)) || (r = dzalloc(&s->tw, n * s->esz)) ||
-      (r = dzalloc(&s->kin[0], n * s->esz)) || (r = dzalloc(&s->kin[1], n * s->esz)) ||
-      (r = dzalloc(&s->kin[2], n * s->esz)) || (r = dzalloc(&s->xa, n * sizeof(double))) ||
+  if ((r = dzalloc(&s->psi, s->n3 * s->esz)) || (r = dzalloc(&s->tw, n * 16)) ||
+      (r = dzalloc(&s->kin[0], n * 16)) || (r = dzalloc(&s->kin[1], n * 16)) ||
+      (r = dzalloc(&s->kin[2], n * 16)) || (r = dzalloc(&s->xa, n * sizeof(double))) ||
       (r = dzalloc(&s->ka, n * sizeof(double))) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double))) ||
       (r = dzalloc(&s->m_z, n * sizeof(double))) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz)) ||
       (r = dzalloc(&s->w_z, n * s->rsz)) || (r = dzalloc(&s->counter, 64)) ||
@@ -611,13 +611,8 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
     return bail(fail(GPE_ERR_HIP, "setup copy failed"));
   const double inv_n3 = 1.0 / double(s->n3);
   const int imag = s->kind == GPE_IMAG ? 1 : 0;
-  if (s->prec == GPE_C64) {
-    launch_twiddle<float>(s->tw, n, s->st);
-    launch_kinetic<float>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
-  } else {
-    launch_twiddle<double>(s->tw, n, s->st);
-    launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
-  }
+  launch_twiddle<double>(s->tw, n, s->st);
+  launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
   if (hipGetLastError() != hipSuccess) return bail(fail(GPE_ERR_HIP, "table kernels failed to launch"));
   if ((r = ensure_red(s, std::max<long long>(kNormBlocks, s->blocks(K_Z_LAST, n))))) return bail(r);
   if ((r = refresh_desc(s))) return bail(r);

@@ -43,6 +43,19 @@ __host__ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
   }
 }
 
+// a * w with the product formed in FP64 and rounded once to R.  Constant
+// factors (twiddles, kinetic phases) stay in FP64, so their rounding does not
+// repeat identically at every step of a complex64 run.
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul_d(cpx<R> a, cpx<double> w) {
+  const double x = a.x, y = a.y;
+  return cpx<R>{R(x * w.x - y * w.y), R(x * w.y + y * w.x)};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cscale_d(cpx<R> a, double s) {
+  return cpx<R>{R(double(a.x) * s), R(double(a.y) * s)};
+}
+
 // ------------------------------------------------------------ static loop --
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -118,27 +131,24 @@ __device__ __forceinline__ void twiddle(cpx<R>& a) {
   } else if constexpr (4 * rr == 3 * m) {
     a = mul_i<-DIR>(a);
   } else if constexpr ((8 * rr) % m == 0) {
-    // odd multiples of pi/4: (c + DIR*i*s) with |c| = |s| = sqrt(1/2)
+    // odd multiples of pi/4: (c + DIR*i*s) with |c| = |s| = sqrt(1/2), FP64
     constexpr double c = ce_trig2pi(rr, m, false);
     constexpr double s = DIR * ce_trig2pi(rr, m, true);
-    constexpr R h = R(0.70710678118654752440084436210484904);
-    constexpr R sc = c > 0 ? h : -h;
-    constexpr R ss = s > 0 ? h : -h;
-    // (x + iy)(sc + i ss) with |sc| = |ss| = h
-    const R p = a.x, q = a.y;
-    if constexpr (sc > 0 && ss > 0) {
-      a = cpx<R>{(p - q) * h, (p + q) * h};
-    } else if constexpr (sc > 0 && ss < 0) {
-      a = cpx<R>{(p + q) * h, (q - p) * h};
-    } else if constexpr (sc < 0 && ss > 0) {
-      a = cpx<R>{-(p + q) * h, (p - q) * h};
+    constexpr double h = 0.70710678118654752440084436210484904;
+    const double p = a.x, q = a.y;
+    if constexpr (c > 0 && s > 0) {
+      a = cpx<R>{R((p - q) * h), R((p + q) * h)};
+    } else if constexpr (c > 0 && s < 0) {
+      a = cpx<R>{R((p + q) * h), R((q - p) * h)};
+    } else if constexpr (c < 0 && s > 0) {
+      a = cpx<R>{R(-(p + q) * h), R((p - q) * h)};
     } else {
-      a = cpx<R>{(q - p) * h, -(p + q) * h};
+      a = cpx<R>{R((q - p) * h), R(-(p + q) * h)};
     }
   } else {
-    constexpr R c = R(ce_trig2pi(rr, m, false));
-    constexpr R s = R(DIR * ce_trig2pi(rr, m, true));
-    a = cpx<R>{a.x * c - a.y * s, a.x * s + a.y * c};
+    constexpr double c = ce_trig2pi(rr, m, false);
+    constexpr double s = DIR * ce_trig2pi(rr, m, true);
+    a = cmul_d(a, cpx<double>{c, s});
   }
 }
 
@@ -174,10 +184,10 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
     v[0] = a + b;
     v[1] = a - b;
   } else if constexpr (M == 3) {
-    constexpr R h = R(0.86602540378443864676372317075293618);  // sqrt(3)/2
+    constexpr double h = 0.86602540378443864676372317075293618;  // sqrt(3)/2
     const cpx<R> a0 = v[0], t = v[1] + v[2], d = v[1] - v[2];
     const cpx<R> m = cpx<R>{a0.x - R(0.5) * t.x, a0.y - R(0.5) * t.y};
-    const cpx<R> r = mul_i<DIR>(cscale(d, h));
+    const cpx<R> r = mul_i<DIR>(cscale_d(d, h));
     v[0] = a0 + t;
     v[1] = m + r;
     v[2] = m - r;

@@ -52,10 +52,10 @@ template <typename R>
 struct SolveDev {
   cpx<R>* psi;
   cpx<R>* work;
-  const cpx<R>* tw;        // tw[r] = exp(-2*pi*i*r/n), r < n (FP64-derived)
-  const cpx<R>* kin_x;     // kinetic factors per axis (1/n^3 folded into x)
-  const cpx<R>* kin_y;
-  const cpx<R>* kin_z;
+  const cpx<double>* tw;   // tw[r] = exp(-2*pi*i*r/n), r < n, FP64
+  const cpx<double>* kin_x;  // kinetic factors per axis, FP64 (1/n^3 folded into x)
+  const cpx<double>* kin_y;
+  const cpx<double>* kin_z;
   const double* karr;      // k axis, FP64
   const double* xarr;      // x axis, FP64
   const R* pxy;            // potential table over (x, y): [slot][n*n]
@@ -98,7 +98,7 @@ struct FastTF {
   static constexpr int THREADS = TP * B;
   // Waves per SIMD the register allocation must allow (LDS caps residency at
   // about four 256-thread blocks per CU, so more registers buy nothing).
-  static constexpr int MIN_WAVES = sizeof(R) == 4 ? 4 : 2;
+  static constexpr int MIN_WAVES = sizeof(R) == 4 ? 3 : 2;
   static constexpr int LSTRIDE = TP + 1;           // padded LDS row
   // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
   // pencils, so an even stride would put them on the same LDS bank.
@@ -111,19 +111,19 @@ struct FastTF {
 
   // Twiddles W_N^{t*k1} staged once per block in LDS as twl[k1*TP + t].
   static constexpr int TWL_ELEMS = M * TP;
-  __device__ static __forceinline__ void stage_twiddles(cpx<R>* twl, const cpx<R>* tw) {
+  __device__ static __forceinline__ void stage_twiddles(cpx<double>* twl, const cpx<double>* tw) {
     for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) twl[e] = gload(tw + (e % TP) * (e / TP));
     __syncthreads();
   }
 
   template <int DIR>
-  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* twl, int) {
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* twl, int) {
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
-      cpx<R> w = twl[k1 * TP + t];
+      cpx<double> w = twl[k1 * TP + t];
       if constexpr (DIR > 0) w.y = -w.y;
-      v[k1] = cmul(v[k1], w);
+      v[k1] = cmul_d(v[k1], w);
     });
     __syncthreads();
     static_for<0, M>([&](auto k1c) {
@@ -157,10 +157,10 @@ struct GenTF {
   __device__ static __forceinline__ int pos(int t, int m) { return t + 64 * m; }
   __device__ static __forceinline__ bool valid(int t, int m, int n) { return t + 64 * m < n; }
   static constexpr int TWL_ELEMS = 1;
-  __device__ static __forceinline__ void stage_twiddles(cpx<R>*, const cpx<R>*) {}
+  __device__ static __forceinline__ void stage_twiddles(cpx<double>*, const cpx<double>*) {}
 
   template <int DIR>
-  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<R>* __restrict__ tw,
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* __restrict__ tw,
                                              int n) {
     __syncthreads();
 #pragma unroll
@@ -174,9 +174,9 @@ struct GenTF {
         cpx<R> acc{R(0), R(0)};
         int idx = 0;
         for (int s = 0; s < n; ++s) {
-          cpx<R> w = gload(tw + idx);
+          cpx<double> w = gload(tw + idx);
           if constexpr (DIR > 0) w.y = -w.y;
-          acc = acc + cmul(lp[s], w);
+          acc = acc + cmul_d(lp[s], w);
           idx += k;
           if (idx >= n) idx -= n;
         }
@@ -287,7 +287,7 @@ __device__ __forceinline__ void load_tile(cpx<R>* v, const cpx<R>* src, long lon
 template <class TF, int AXIS, int KIND, typename R>
 __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R> a) {
   __shared__ cpx<R> lds[TF::LDS_ELEMS];
-  __shared__ cpx<R> twl_lds[TF::TWL_ELEMS];
+  __shared__ cpx<double> twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
   constexpr int M = TF::M;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const int ntiles = tile_count<TF, AXIS>(n);
   cpx<R>* lp = lds + b * TF::PS;
   TF::stage_twiddles(twl_lds, s.tw);
-  const cpx<R>* twp = TF::kFast ? static_cast<const cpx<R>*>(twl_lds) : s.tw;
+  const cpx<double>* twp = TF::kFast ? static_cast<const cpx<double>*>(twl_lds) : s.tw;
   const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
   const bool has_vim = s.vim != nullptr;
@@ -341,16 +341,16 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     } else if constexpr (KIND == K_X_KIN) {
       TF::template fft<-1>(v, lp, t, twp, n);
       if (s.imag) {
-        const R eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
+        const double eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           if (TF::valid(t, m, n)) {
-            const R f = gload(s.kin_x + TF::pos(t, m)).x * eyz;
-            v[m] = cpx<R>{v[m].x * f, v[m].y * f};
+            const double f = gload(s.kin_x + TF::pos(t, m)).x * eyz;
+            v[m] = cscale_d(v[m], f);
           }
         }
       } else {
-        const cpx<R> eyz = cmul(gload(s.kin_y + pj), gload(s.kin_z + pk));
+        const cpx<double> eyz = cmul(gload(s.kin_y + pj), gload(s.kin_z + pk));
         double ky2 = 0.0, kz2 = 0.0;
         if (s.masked) {
           ky2 = __dmul_rn(gload(s.karr + pj), gload(s.karr + pj));
@@ -360,14 +360,14 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         for (int m = 0; m < M; ++m) {
           if (TF::valid(t, m, n)) {
             const int i = TF::pos(t, m);
-            cpx<R> f = cmul(gload(s.kin_x + i), eyz);
+            cpx<double> f = cmul(gload(s.kin_x + i), eyz);
             if (s.masked) {
               const double kx = gload(s.karr + i);
               // same association as kx*kx + ky*ky + kz*kz, without contraction
               const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
-              if (k2 > s.kcut2) f = cpx<R>{R(0), R(0)};
+              if (k2 > s.kcut2) f = cpx<double>{0.0, 0.0};
             }
-            v[m] = cmul(v[m], f);
+            v[m] = cmul_d(v[m], f);
           }
         }
       }

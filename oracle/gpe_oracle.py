@@ -26,6 +26,22 @@ import numpy as np
 A0 = 5.29177210544e-11
 LENGTH_UNIT = 1e-6
 
+# FFT backend: numpy.fft (single-threaded pocketfft) by default; tests may
+# switch to scipy.fft with worker threads for speed (same algorithm family,
+# FP64 throughout).
+_fftn, _ifftn = np.fft.fftn, np.fft.ifftn
+
+
+def use_threads(workers):
+    """Run the oracle's 3D FFTs on `workers` threads (None: numpy, 1 thread)."""
+    global _fftn, _ifftn
+    if workers is None:
+        _fftn, _ifftn = np.fft.fftn, np.fft.ifftn
+        return
+    import scipy.fft
+    _fftn = lambda a: scipy.fft.fftn(a, workers=workers)  # noqa: E731
+    _ifftn = lambda a: scipy.fft.ifftn(a, workers=workers)  # noqa: E731
+
 
 def get_grids(n, length):
     """x = linspace(-L/2, L/2, N, endpoint=False); k = 2*pi*fftfreq(N, L/N).
@@ -163,12 +179,12 @@ class Problem:
                 t_mid = float(t) + dt / 2
                 potential = potential + self.W(t_mid)
             psi = real_half_step(psi, self.v_static, potential, self.beta, dt)
-            psi_k = kinetic(np.fft.fftn(psi), self.k, dt, self.kcut)
-            psi = np.fft.ifftn(psi_k)
+            psi_k = kinetic(_fftn(psi), self.k, dt, self.kcut)
+            psi = _ifftn(psi_k)
             return real_half_step(psi, self.v_static, potential, self.beta, dt)
         psi = imag_half_step(psi, self.v_static, self.U, self.beta, dt)
-        psi_k = imag_kinetic(np.fft.fftn(psi), self.k, dt)
-        psi = np.fft.ifftn(psi_k)
+        psi_k = imag_kinetic(_fftn(psi), self.k, dt)
+        psi = _ifftn(psi_k)
         psi = imag_half_step(psi, self.v_static, self.U, self.beta, dt)
         return normalise(psi, self.x)
 
@@ -194,7 +210,7 @@ class Problem:
         potential = self.U + (self.v_static.real if np.iscomplexobj(self.v_static) else self.v_static)
         if self.dynamic_source:
             potential = potential + self.W(t)
-        F = np.fft.fftn(psi)
+        F = _fftn(psi)
         f2 = F.real ** 2 + F.imag ** 2
         norm = dx3 * float(np.sum(a2))
         ekin = float(np.sum(f2 * k_squared(self.k))) * dx ** 3 / psi.size

@@ -1,0 +1,209 @@
+"""The drop-in API on the device: the reference's closed-form propagator tests
+(tests/test_propagators.py:32-200 of the reference) restated through this
+package's stage / parse_yaml / launch_stage / main."""
+
+import csv
+import json
+import sys
+from pathlib import Path
+from unittest.mock import patch
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * np.pi
+
+
+@pytest.fixture
+def env(tmp_path, require_gpu):
+    from gpe_simulator_amd.configuration import dump_yaml, new_config
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+
+    config = new_config()
+    config["simulation"].update(N_grids=8, simulation_range=TWO_PI, data_root=str(tmp_path / "data"),
+                                monitor_root=str(tmp_path / "monitor"))
+    settings = config["stages"][0]
+    settings["box"]["enabled"] = False
+    settings["cloud"]["a"] = 0
+    settings["evolution"].update(steps=3, dt=0.01)
+    settings["save"].update(scalar_interval=0, save_final=False)
+    settings["monitor"] = dict.fromkeys(settings["monitor"], 0)
+
+    class Env:
+        pass
+
+    e = Env()
+    e.config, e.root = config, tmp_path
+
+    def stage(precision="complex128"):
+        path = tmp_path / "test.yaml"
+        path.write_text(dump_yaml(config))
+        item = parse_yaml(path, precision=precision)[0]
+        item.prepare()
+        return item
+
+    e.stage = stage
+    return e
+
+
+def plane_wave(item, mode=1):
+    wave = np.exp(1j * mode * item.x_array)[:, None, None]
+    return np.broadcast_to(wave, (item.N_grids,) * 3).copy() / TWO_PI ** 1.5
+
+
+@pytest.mark.parametrize("precision,rtol", [("complex128", 1e-11), ("complex64", 3e-6)])
+def test_real_phase_for_all_potential_and_cutoff_combinations(env, precision, rtol):
+    from gpe_simulator_amd.helper import get_N
+    from gpe_simulator_amd.launch import launch_stage
+    settings = env.config["stages"][0]
+    settings["evolution"]["type"] = "real"
+    settings["cloud"]["a"] = 10
+    for static in (False, True):
+        for dynamic in (False, True):
+            for cutoff in (False, True):
+                settings["potential"]["static"].update(enabled=static, expression="1/2")
+                settings["potential"]["dynamic"].update(enabled=dynamic, expression="t/4")
+                settings["k_cut"].update(enabled=cutoff, value=1.5)
+                item = env.stage(precision)
+                initial = plane_wave(item)
+                item.psi = initial
+                time = item.steps * item.dt
+                phase = (1 + item.beta / TWO_PI ** 3) * time
+                phase += (0.5 * time if static else 0) + (time ** 2 / 8 if dynamic else 0)
+                result = launch_stage(item)
+                np.testing.assert_allclose(np.asarray(result), initial * np.exp(-1j * phase), rtol=rtol,
+                                           atol=rtol * 1e-1)
+                assert get_N(result) == pytest.approx(1, abs=10 * rtol)
+                item.release()
+
+
+def test_per_step_propagator_matches_batched(env):
+    """Calling the propagator once per step (reference style) and launch_stage's
+    batched calls give the same state."""
+    from gpe_simulator_amd.launch import launch_stage
+    settings = env.config["stages"][0]
+    settings["evolution"].update(type="real", steps=5)
+    settings["cloud"]["a"] = 10
+    settings["potential"]["static"].update(enabled=True, expression="x**2/3 + cos(y)")
+    settings["potential"]["dynamic"].update(enabled=True, expression="sin(t)*z")
+    a, b = env.stage(), env.stage()
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=(8, 8, 8)) + 1j * rng.normal(size=(8, 8, 8))
+    # normalised, so the nonlinear phase per step (beta |psi|^2 dt) stays small
+    # and rounding differences are not amplified by the dynamics
+    psi /= np.sqrt(np.sum(np.abs(psi) ** 2) * (TWO_PI / 8) ** 3)
+    a.psi = psi
+    b.psi = psi
+    for i in range(5):
+        a.psi = a.propagator(a, i * a.dt)
+    launch_stage(b)
+    np.testing.assert_allclose(np.asarray(a.psi), np.asarray(b.psi), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("static", [False, True])
+def test_imaginary_decay_and_normalization(env, static):
+    from gpe_simulator_amd.launch import launch_stage
+    env.config["stages"][0]["potential"]["static"].update(enabled=static, expression="1/2")
+    item = env.stage()
+    low, high = plane_wave(item, 1), plane_wave(item, 2)
+    item.psi = low + high
+    time = item.steps * item.dt
+    expected = low * np.exp(-time) + high * np.exp(-4 * time)
+    dx = float(item.x_array[1] - item.x_array[0])
+    expected /= np.sqrt(np.sum(np.abs(expected) ** 2) * dx ** 3)
+    np.testing.assert_allclose(np.asarray(launch_stage(item)), expected, rtol=1e-11, atol=1e-12)
+
+
+def test_absorber_keeps_analytical_norm_loss(env):
+    from gpe_simulator_amd.helper import get_N
+    from gpe_simulator_amd.launch import launch_stage
+    env.config["stages"][0]["evolution"]["type"] = "real"
+    item = env.stage()
+    initial = plane_wave(item)
+    item.psi = initial
+    gamma = 0.4
+    item.staticV.fill(-1j * gamma)
+    time = item.steps * item.dt
+    result = launch_stage(item)
+    np.testing.assert_allclose(np.asarray(result), initial * np.exp((-gamma - 1j) * time), rtol=1e-11, atol=1e-12)
+    assert get_N(result) == pytest.approx(np.exp(-2 * gamma * time), abs=1e-11)
+
+
+def test_cutoff_removes_high_mode_without_renormalizing(env):
+    from gpe_simulator_amd.helper import get_N
+    from gpe_simulator_amd.launch import launch_stage
+    settings = env.config["stages"][0]
+    settings["evolution"]["type"] = "real"
+    settings["k_cut"].update(enabled=True, value=1.5)
+    item = env.stage()
+    low, high = plane_wave(item, 1), plane_wave(item, 2)
+    item.psi = (low + high) / np.sqrt(2)
+    result = launch_stage(item)
+    expected = low * np.exp(-1j * item.steps * item.dt) / np.sqrt(2)
+    np.testing.assert_allclose(np.asarray(result), expected, rtol=1e-11, atol=1e-12)
+    assert get_N(result) == pytest.approx(0.5, abs=1e-11)
+
+
+def test_snapshot_roundtrip_across_slab_boundary(env):
+    from gpe_simulator_amd.state_io import load_wavefunction, save_wavefunction
+    env.config["simulation"]["N_grids"] = 18
+    item = env.stage()
+    rng = np.random.default_rng(5)
+    original = rng.normal(size=(18,) * 3) + 1j * rng.normal(size=(18,) * 3)
+    item.psi = original
+    path = env.root / "state.npy"
+    save_wavefunction(path, item, 0)
+    np.testing.assert_array_equal(np.load(path), original)
+    item.psi = np.zeros((18,) * 3)
+    field = load_wavefunction(path, item)
+    np.testing.assert_array_equal(np.asarray(field), original)
+
+
+def test_angular_momentum_of_vortex(require_gpu):
+    """Lz = +-N for (x +- i y) exp(-r^2/2) (tests/test_diagnostics.py:42-49)."""
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.grids import get_grids
+    x, k = get_grids(48, 16)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    for charge in (-1, 1):
+        psi = (X + charge * 1j * Y) * np.exp(-(X * X + Y * Y + Z * Z) / 2)
+        solve = make_solve(48, x, k, "real", 1e-3, 0.0)
+        solve.upload(psi)
+        norm = solve.norm()
+        assert solve.angular_momentum() == pytest.approx(charge * norm, rel=1e-9, abs=1e-10)
+
+
+def test_yaml_run_with_two_stages_and_all_saved_diagnostics(env):
+    from gpe_simulator_amd.configuration import dump_yaml
+    from gpe_simulator_amd.main import main
+    first = env.config["stages"][0]
+    first["evolution"]["steps"] = 2
+    first["save"].update(save_final=True, scalar_interval=1, nk_interval=1)
+    first["monitor"].update({name: 1 for name in first["monitor"] if name.startswith("show_")})
+    second = json.loads(json.dumps(first))
+    second["evolution"]["type"] = "real"
+    second["initial_state"]["type"] = "continue"
+    second["box"].update(enabled=True, R=1, L=2, wall_thickness=0.5, trap_depth=0.3)
+    second["noise"]["enabled"] = True
+    second["absorber"].update(enabled=True, activation_distance=0.5, strength=0.2)
+    second["potential"]["static"].update(enabled=True, expression="x**2/20")
+    second["potential"]["dynamic"].update(enabled=True, expression="cos(t)*y/10")
+    second["k_cut"].update(enabled=True, value=2)
+    env.config["stages"].append(second)
+    path = env.root / "stages.yaml"
+    path.write_text(dump_yaml(env.config))
+    with patch("builtins.print"):
+        main([str(path)])
+    states = sorted((env.root / "data").glob("*/stage_*/psi_*.npy"))
+    assert len(states) == 2
+    for state in states:
+        saved = np.load(state)
+        assert saved.dtype == np.complex128 and np.isfinite(saved).all()
+    scalar_files = list((env.root / "monitor").glob("*/stage_*/scalars.csv"))
+    assert len(scalar_files) == 2
+    for scalar in scalar_files:
+        with scalar.open() as file:
+            assert [row["step"] for row in csv.DictReader(file)] == ["1", "2"]
+        assert len(list(scalar.parent.glob("*_00000002.npz"))) == 8

@@ -1,0 +1,127 @@
+"""Long-run parity against the FP64 oracle and precision checks on the device.
+
+Bars (stated per test): complex128 psi within relative L2 1e-10 of the oracle
+after 1000 steps, mu and E within 1e-11 relative; complex64 + FP64 reductions
+mu within 1e-6 relative of the FP64 result.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as orc
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+
+@pytest.fixture(autouse=True, scope="module")
+def threaded_oracle():
+    orc.use_threads(8)
+    yield
+    orc.use_threads(None)
+
+BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def engine(problem, precision, kind=None):
+    from gpe_simulator_amd.engine import make_solve
+    return make_solve(problem.n, problem.x, problem.k, kind or problem.kind, problem.dt, problem.beta,
+                      static=problem.static_source, dynamic=problem.dynamic_source, precision=precision)
+
+
+def test_imaginary_ground_state_1000_steps_c128(require_gpu):
+    """64^3 on [-10,10)^3, V = x^2+y^2+z^2, beta = 200, Gaussian start,
+    dtau = 5e-4, renormalised every step (the CPU-runnable configuration in
+    reference units), 1000 steps."""
+    p = orc.Problem(64, 20.0, "imaginary", 5e-4, 200.0, static="x**2 + y**2 + z**2")
+    psi0 = orc.gaussian(p.x, 1.0)
+    expected = p.run(psi0, 1000)
+    solve = engine(p, "c128")
+    solve.gaussian(1.0)
+    solve.step(1000)
+    got = solve.to_numpy()
+    assert rel_l2(got, expected) < 1e-10
+    e_ref, e_dev = p.energies(expected), solve.diagnostics(0.0, spectrum=False)
+    assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-11)
+    assert e_dev["E"] == pytest.approx(e_ref["E"], rel=1e-11)
+    assert e_dev["norm"] == pytest.approx(1.0, abs=1e-13)
+
+
+def test_stirring_1000_steps_c128(require_gpu):
+    """Real-time stirring at 64^3 with the moving Gaussian beam, 1000 steps."""
+    from gpe_simulator_amd.engine import midpoint_times
+    p = orc.Problem(64, 16.0, "real", 5e-4, 400.0, static="x**2 + y**2 + z**2", dynamic=BEAM)
+    psi0 = orc.synthetic_state(64, 16.0, seed=11, beta=400.0)
+    expected = p.run(psi0, 1000)
+    solve = engine(p, "c128")
+    solve.upload(psi0)
+    solve.step(1000, midpoint_times(0, 1000, p.dt))
+    got = solve.to_numpy()
+    assert rel_l2(got, expected) < 1e-10
+    t = 1000 * p.dt
+    e_ref, e_dev = p.energies(expected, t), solve.diagnostics(t, spectrum=False)
+    assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-11)
+    assert e_dev["E"] == pytest.approx(e_ref["E"], rel=1e-11)
+    # real time conserves the norm (no absorber)
+    assert e_dev["norm"] == pytest.approx(orc.get_N(psi0, p.x), rel=1e-12)
+
+
+def test_c64_mu_against_fp64_at_128(require_gpu):
+    """complex64 field + FP64 reductions: mu within 1e-6 of the FP64 oracle
+    after 100 stirring steps at 128^3."""
+    from gpe_simulator_amd.engine import midpoint_times
+    p = orc.Problem(128, 24.0, "real", 5e-4, 1000.0, static="x**2 + y**2 + 16*z**2", dynamic=BEAM)
+    psi0 = orc.synthetic_state(128, 24.0, seed=2, beta=1000.0,
+                               potential=p.U)
+    expected = p.run(psi0, 100)
+    solve = engine(p, "c64")
+    solve.upload(psi0)
+    solve.step(100, midpoint_times(0, 100, p.dt))
+    t = 100 * p.dt
+    mu_ref = p.energies(expected, t)["mu"]
+    diag = solve.diagnostics(t, spectrum=False)
+    assert abs(diag["mu"] - mu_ref) / abs(mu_ref) < 1e-6
+    assert rel_l2(solve.to_numpy(), expected) < 1e-4
+
+
+def test_headline_256_c64_vs_c128(require_gpu):
+    """256^3 stirring run: c64 and c128 engines agree on mu to 1e-6 and the
+    c128 norm is conserved to 1e-12 over 200 steps; c128 is parity-tested
+    against the oracle above."""
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    x, k = orc.get_grids(256, 40.0)
+    psi0 = orc.synthetic_state(256, 40.0, seed=0, beta=4000.0)
+    out = {}
+    for precision in ("c64", "c128"):
+        solve = make_solve(256, x, k, "real", 2.5e-4, 4000.0, static="x**2 + y**2 + z**2", dynamic=BEAM,
+                           precision=precision)
+        solve.upload(psi0)
+        n0 = solve.norm()
+        solve.step(200, midpoint_times(0, 200, 2.5e-4))
+        out[precision] = (solve.diagnostics(200 * 2.5e-4, spectrum=False), n0)
+        solve.close()
+    d64, d128 = out["c64"][0], out["c128"][0]
+    assert abs(d64["mu"] - d128["mu"]) / abs(d128["mu"]) < 1e-6
+    assert d128["norm"] == pytest.approx(out["c128"][1], rel=1e-12)
+    assert d64["norm"] == pytest.approx(out["c64"][1], rel=1e-6)
+
+
+def test_time_reversibility_256_c128(require_gpu):
+    """Full-size property: S(-dt) S(dt) psi = psi for the real potential."""
+    from gpe_simulator_amd.engine import make_solve
+    x, k = orc.get_grids(256, 40.0)
+    psi0 = orc.synthetic_state(256, 40.0, seed=5, beta=4000.0)
+    fwd = make_solve(256, x, k, "real", 2.5e-4, 4000.0, static="x**2 + y**2 + z**2", dynamic=BEAM)
+    fwd.upload(psi0)
+    fwd.step(1, np.array([0.3 + 1.25e-4]))
+    mid = fwd.to_numpy()
+    fwd.close()
+    # For a real potential conj o S(dt) o conj = S(-dt) with the same midpoint
+    # time, and S(-dt) S(dt) is the identity.
+    back = make_solve(256, x, k, "real", 2.5e-4, 4000.0, static="x**2 + y**2 + z**2", dynamic=BEAM)
+    back.upload(np.conj(mid))
+    back.step(1, np.array([0.3 + 1.25e-4]))
+    np.testing.assert_allclose(np.conj(back.to_numpy()), psi0, rtol=0, atol=1e-12 * np.abs(psi0).max())

@@ -1,0 +1,179 @@
+"""C-ABI library and workflow plumbing (CPU, no device calls).
+
+The library test checks that libgpe_hip.so loads and exports every function
+include/gpe_hip.h declares.  The workflow tests mirror the reference's CPU
+tests of the stage driver (tests/test_workflow.py of the reference): parsing
+defers device work and shares axes, the stage clock is t_i = i*dt and resets
+per stage, output folders follow the same rules, and --check never loads the
+engine.
+"""
+
+import json
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "gpe_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(gpe_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def library():
+    from gpe_simulator_amd import _lib
+    if not _lib.LIBRARY.exists():
+        from gpe_simulator_amd import build
+        build.build()
+    return _lib
+
+
+def test_library_exports_every_declared_function(library):
+    names = header_functions()
+    assert len(names) >= 20
+    handle = library.lib()
+    for name in names:
+        assert hasattr(handle, name), name
+    assert sorted(library.EXPORTS) == names
+    assert handle.gpe_version().startswith(b"gpe_hip")
+
+
+def test_library_reports_errors_without_device(library):
+    handle = library.lib()
+    assert handle.gpe_last_error() is not None
+    # a null context is rejected before any device work
+    assert handle.gpe_mem_info(None, None, None) == library.GPE_ERR_INVALID
+    assert b"null" in handle.gpe_last_error()
+
+
+def test_library_is_gfx950_code():
+    lib = (ROOT / "gpe_simulator_amd" / "libgpe_hip.so")
+    if not lib.exists():
+        pytest.skip("library not built")
+    assert b"gfx950" in lib.read_bytes()
+
+
+# ------------------------------------------------------------ workflow --
+def write_config(root, stages=1, **first_updates):
+    from gpe_simulator_amd.configuration import dump_yaml, new_config
+    config = new_config()
+    config["simulation"].update(N_grids=4, simulation_range=8, data_root=str(root / "data"),
+                                monitor_root=str(root / "monitor"))
+    first = config["stages"][0]
+    first["evolution"].update(type="real", steps=6, dt=0.125)
+    first["box"]["enabled"] = False
+    first["monitor"] = dict.fromkeys(first["monitor"], 0)
+    first["save"].update(scalar_interval=0, save_final=False)
+    for key, value in first_updates.items():
+        first[key].update(value)
+    for _ in range(stages - 1):
+        second = json.loads(json.dumps(first))
+        second["initial_state"]["type"] = "continue"
+        config["stages"].append(second)
+    path = root / "experiment.yaml"
+    path.write_text(dump_yaml(config))
+    return path, config
+
+
+def test_parser_shares_axes_and_defers_device_work(tmp_path):
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, config = write_config(tmp_path, stages=2, potential={"static": {"enabled": True,
+                                                                          "expression": "1/2*y**2"}})
+    stages = parse_yaml(path)
+    assert stages[0].x_array is stages[1].x_array and stages[0].k_array is stages[1].k_array
+    for item in stages:
+        assert item.psi is None and item.solve is None and item.propagator is None
+        assert not item.data_dir.exists()
+    assert stages[0].potential_params["static"]["expression"] == "((1.0 / 2.0) * pow(y, 2.0))"
+    assert stages[0].data_dir.parent.name == stages[1].monitor_dir.parent.name
+    assert stages[0].precision == "complex128"
+    assert parse_yaml(path, precision="complex64")[0].precision == "complex64"
+
+
+def test_duplicate_and_escaping_output_folders(tmp_path):
+    from gpe_simulator_amd.configuration import dump_yaml
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, config = write_config(tmp_path)
+    first = config["stages"][0]
+    first["save"]["directory"] = "same_folder"
+    second = json.loads(json.dumps(first))
+    second["save"]["directory"] = "nested/../same_folder"
+    config["stages"].append(second)
+    path.write_text(dump_yaml(config))
+    with pytest.raises(ValueError, match="reuses another stage's output directory"):
+        parse_yaml(path)
+    config["stages"] = [first]
+    first["save"]["directory"] = "../shared"
+    path.write_text(dump_yaml(config))
+    with pytest.raises(ValueError, match="stay inside the run folder"):
+        parse_yaml(path)
+
+
+def test_stage_clock_and_per_step_propagator(tmp_path):
+    """A propagator without `advance` is called once per step with t = i*dt,
+    and the clock restarts in each stage (tests/test_workflow.py:235-272)."""
+    from gpe_simulator_amd.launch import launch_stage
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, _ = write_config(tmp_path, stages=2)
+    stages = parse_yaml(path)
+    stages[1].potential_params["dynamic"]["activate"] = True
+    times = []
+
+    def step(current, t=None):
+        times.append(t)
+        return current.psi * 0.5
+
+    for item in stages:
+        item.steps = 2
+        item.psi = np.full((4, 4, 4), 0.1 + 0.2j)
+        item.propagator = step
+        launch_stage(item)
+    assert times == [None, None, 0, 0.125]
+
+
+def test_launch_stage_batches_steps_between_due_outputs(tmp_path):
+    from gpe_simulator_amd import launch
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, _ = write_config(tmp_path)
+    item = parse_yaml(path)[0]
+    item.steps = 23
+    item.save_params.update(scalar_interval=5, nk_interval=0, psi_interval=0, save_final=False)
+    item.monitor_params["show_nk_interval"] = 7
+    calls, saved = [], []
+
+    def step(current):
+        raise AssertionError("the batched path should be used")
+
+    def advance(current, first, count):
+        calls.append((first, count))
+        return current.psi
+
+    step.advance = advance
+    item.propagator = step
+    item.psi = np.zeros((4, 4, 4), complex)
+    original = launch.save_outputs
+    launch.save_outputs = lambda stage, i, **kw: saved.append(i)
+    try:
+        launch.launch_stage(item)
+    finally:
+        launch.save_outputs = original
+    assert calls == [(0, 5), (5, 2), (7, 3), (10, 4), (14, 1), (15, 5), (20, 1), (21, 2)]
+    assert saved == [0, 5, 7, 10, 14, 15, 20, 21, 23]
+
+
+def test_cli_check_does_not_load_the_engine(tmp_path):
+    path, _ = write_config(tmp_path)
+    code = ("import sys, runpy; sys.argv=['main', sys.argv[1], '--check']; "
+            "runpy.run_module('gpe_simulator_amd.main', run_name='__main__'); "
+            "print('ENGINE' if 'gpe_simulator_amd._lib' in sys.modules else 'NOENGINE')")
+    out = subprocess.run([sys.executable, "-c", code, str(path)], capture_output=True, text=True, cwd=ROOT,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Valid configuration: 1 stages" in out.stdout
+    assert "NOENGINE" in out.stdout

@@ -1,0 +1,252 @@
+"""Pins of the FP64 oracle (CPU).
+
+Each check is a closed form, an invariant or a brute-force computation that
+the reference's own tests use (tests/test_propagators.py:61-158,
+tests/test_diagnostics.py:31-81 of the reference) or that the mathematics
+fixes; a plausible slip in the oracle (a missing 1/2 on dt, k^2 with a 1/2, a
+wrong sign in the nonlinear phase, a midpoint time off by dt/2, a shifted
+shell) fails at least one of them.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as orc
+
+TWO_PI = 2 * np.pi
+
+
+def plane_wave(x, mode=1):
+    n = len(x)
+    wave = np.exp(1j * mode * x)[:, None, None]
+    return np.broadcast_to(wave, (n,) * 3).copy() / TWO_PI ** 1.5
+
+
+@pytest.mark.parametrize("static", [False, True])
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("cutoff", [False, True])
+def test_real_phase_closed_form(static, dynamic, cutoff):
+    # tests/test_propagators.py:61-84: N=8, L=2pi, a=10, 3 steps of dt=0.01.
+    beta = orc.get_beta(150000, 10)
+    p = orc.Problem(8, TWO_PI, "real", 0.01, beta, static="1/2" if static else None,
+                    dynamic="t/4" if dynamic else None, kcut=1.5 if cutoff else None)
+    psi0 = plane_wave(p.x)
+    result = p.run(psi0, 3)
+    T = 3 * 0.01
+    phase = (1 + beta / TWO_PI ** 3) * T + (0.5 * T if static else 0) + (T ** 2 / 8 if dynamic else 0)
+    np.testing.assert_allclose(result, psi0 * np.exp(-1j * phase), rtol=1e-11, atol=1e-12)
+    assert orc.get_N(result, p.x) == pytest.approx(1, abs=1e-11)
+
+
+@pytest.mark.parametrize("kind,dynamic", [("imaginary", False), ("real", False), ("real", True)])
+def test_expression_half_step_at_each_point(kind, dynamic):
+    # tests/test_propagators.py:86-112: odd N=7, psi = 0.2+0.1i, beta = 0.
+    x, _ = orc.get_grids(7, TWO_PI)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    potential = X / 2 + Y ** 2 / 10 + np.abs(Z) / 5
+    t = 0.37
+    U = orc.eval_expression("x/2 + y**2/10 + abs(z)/5", x)
+    if dynamic:
+        U = U + orc.eval_expression("sin(t)*x + cos(pi)*z", x, t)
+        potential = potential + np.sin(t) * X - Z
+    psi = np.full((7,) * 3, 0.2 + 0.1j)
+    dt = 0.01
+    if kind == "imaginary":
+        got = orc.imag_half_step(psi, np.zeros((7,) * 3), U, 0.0, dt)
+        expected = (0.2 + 0.1j) * np.exp(-potential * dt / 2)
+    else:
+        got = orc.real_half_step(psi, np.zeros((7,) * 3, complex), U, 0.0, dt)
+        expected = (0.2 + 0.1j) * np.exp(-1j * potential * dt / 2)
+    np.testing.assert_allclose(got, expected, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("static", [False, True])
+def test_imaginary_decay_and_normalisation(static):
+    # tests/test_propagators.py:114-129
+    p = orc.Problem(8, TWO_PI, "imaginary", 0.01, 0.0, static="1/2" if static else None)
+    low, high = plane_wave(p.x, 1), plane_wave(p.x, 2)
+    result = p.run(low + high, 3)
+    T = 0.03
+    expected = low * np.exp(-T) + high * np.exp(-4 * T)
+    dx = p.x[1] - p.x[0]
+    expected /= np.sqrt(np.sum(np.abs(expected) ** 2) * dx ** 3)
+    np.testing.assert_allclose(result, expected, rtol=1e-11, atol=1e-12)
+
+
+def test_absorber_norm_loss():
+    # tests/test_propagators.py:131-144
+    gamma = 0.4
+    v = np.full((8,) * 3, -1j * gamma)
+    p = orc.Problem(8, TWO_PI, "real", 0.01, 0.0, v_static=v)
+    psi0 = plane_wave(p.x)
+    result = p.run(psi0, 3)
+    T = 0.03
+    np.testing.assert_allclose(result, psi0 * np.exp((-gamma - 1j) * T), rtol=1e-11, atol=1e-12)
+    assert orc.get_N(result, p.x) == pytest.approx(np.exp(-2 * gamma * T), abs=1e-11)
+
+
+def test_cutoff_removes_high_mode():
+    # tests/test_propagators.py:146-158
+    p = orc.Problem(8, TWO_PI, "real", 0.01, 0.0, kcut=1.5)
+    low, high = plane_wave(p.x, 1), plane_wave(p.x, 2)
+    result = p.run((low + high) / np.sqrt(2), 3)
+    np.testing.assert_allclose(result, low * np.exp(-1j * 0.03) / np.sqrt(2), rtol=1e-11, atol=1e-12)
+    assert orc.get_N(result, p.x) == pytest.approx(0.5, abs=1e-11)
+
+
+def test_norm_and_kinetic_energy_of_plane_wave():
+    # tests/test_diagnostics.py:31-39: mode (2, -3, 1) has k^2 = 14.
+    x, k = orc.get_grids(16, TWO_PI)
+    phase = 2 * x[:, None, None] - 3 * x[None, :, None] + x[None, None, :]
+    psi = np.exp(1j * phase) / TWO_PI ** 1.5
+    assert orc.get_N(psi, x) == pytest.approx(1, abs=1e-12)
+    assert orc.get_E(np.fft.fftn(psi), x, k) == pytest.approx(14, abs=1e-11)
+
+
+@pytest.mark.parametrize("n", [2, 7, 8, 18])
+def test_shells_counts_and_parseval(n):
+    # tests/test_diagnostics.py:52-72, with brute-force shell indices.
+    x, k = orc.get_grids(n, TWO_PI)
+    rng = np.random.default_rng(42)
+    psi = rng.normal(size=(n,) * 3) + 1j * rng.normal(size=(n,) * 3)
+    spectrum = orc.nk_extraction(np.fft.fftn(psi), x, k)
+    dk = abs(k[1] - k[0])
+    counts = {}
+    for a in k:
+        for b in k:
+            for c in k:
+                s = int(math.floor(math.sqrt(a * a + b * b + c * c) / dk + 0.5))
+                counts[s] = counts.get(s, 0) + 1
+    brute = np.zeros(len(spectrum["k"]), dtype=np.int64)
+    for s, c in counts.items():
+        brute[s] = c
+    np.testing.assert_array_equal(spectrum["counts"], brute)
+    assert spectrum["counts"].sum() == n ** 3
+    assert spectrum["Nk"].sum() == pytest.approx(orc.get_N(psi, x), rel=1e-12)
+    assert spectrum["edges"][0] == 0
+    np.testing.assert_array_equal(spectrum["complete_shells"], spectrum["edges"][1:] <= n * dk / 2)
+
+
+def test_zero_mode_spectrum():
+    # tests/test_diagnostics.py:75-81
+    x, k = orc.get_grids(8, TWO_PI)
+    psi = np.ones((8,) * 3, dtype=np.complex128) / TWO_PI ** 1.5
+    spectrum = orc.nk_extraction(np.fft.fftn(psi), x, k)
+    assert spectrum["Nk"][0] == pytest.approx(1)
+    np.testing.assert_allclose(spectrum["Nk"][1:], 0, atol=1e-25)
+    assert spectrum["counts"][0] == 1
+
+
+@pytest.mark.parametrize("n", [2, 7, 8, 18, 64, 96, 128, 192, 256])
+@pytest.mark.parametrize("length", [TWO_PI, 20.0, 24.0, 40.0, 70.0])
+def test_integer_shell_identity(n, length):
+    """floor(|k|/dk + 1/2) from FP64 axes equals m + (s > m^2 + m), m = isqrt(s)."""
+    x, k = orc.get_grids(n, length)
+    dk = abs(float(k[1] - k[0]))
+    p = np.rint(k / dk).astype(np.int64)
+    assert np.all(np.abs(p * dk - k) < 1e-9 * max(1.0, dk))
+    P, Q, R = np.meshgrid(p, p, p, indexing="ij")
+    fp64 = np.floor(np.sqrt(orc.k_squared(k)) / dk + 0.5).astype(np.int64)
+    np.testing.assert_array_equal(orc.shell_index_integer(P, Q, R), fp64)
+
+
+def test_fft_convention_brute_force():
+    """numpy.fft.fftn is the unnormalised forward DFT e^{-2 pi i n k / N}."""
+    rng = np.random.default_rng(1)
+    n = 4
+    a = rng.normal(size=(n,) * 3) + 1j * rng.normal(size=(n,) * 3)
+    idx = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(idx, idx) / n)
+    brute = np.einsum("ia,jb,kc,abc->ijk", W, W, W, a)
+    np.testing.assert_allclose(np.fft.fftn(a), brute, rtol=1e-12, atol=1e-12)
+
+
+def test_time_reversibility():
+    """S(-dt) S(dt) psi = psi for a real potential without cutoff."""
+    n, length, beta = 16, 10.0, 40.0
+    fwd = orc.Problem(n, length, "real", 2e-3, beta, static="x**2 + y**2/2", dynamic="cos(t)*x")
+    bwd = orc.Problem(n, length, "real", -2e-3, beta, static="x**2 + y**2/2", dynamic="cos(t)*x")
+    psi0 = orc.synthetic_state(n, length, seed=4, beta=beta)
+    t = 0.3
+    one = fwd.step(psi0.copy(), t)
+    # the forward step used t_mid = t + dt/2; the backward step from t + dt uses
+    # (t + dt) + (-dt)/2, the same midpoint
+    back = bwd.step(one.copy(), t + 2e-3)
+    np.testing.assert_allclose(back, psi0, rtol=1e-12, atol=1e-13)
+
+
+def test_stage_clock_uses_index_times_dt():
+    """launch_stage passes t_i = i*dt; the oracle's run uses the same times
+    (tests/test_workflow.py:262 pins [0, 0.125] for a second stage)."""
+    seen = []
+    p = orc.Problem(4, 8.0, "real", 0.125, 0.0, dynamic="t*y")
+    original = p.W
+
+    def spy(t):
+        seen.append(t)
+        return original(t)
+
+    p.W = spy
+    p.run(np.ones((4,) * 3, complex), 2)
+    assert seen == [0 * 0.125 + 0.125 / 2, 1 * 0.125 + 0.125 / 2]
+
+
+def test_harmonic_ground_state_energy_and_mu():
+    """beta = 0, V = r^2: pi^{-3/4} exp(-r^2/2) has E = mu = 3 (reference units)."""
+    p = orc.Problem(64, 20.0, "imaginary", 1e-3, 0.0, static="x**2 + y**2 + z**2")
+    X, Y, Z = p.x[:, None, None], p.x[None, :, None], p.x[None, None, :]
+    psi = (np.pi ** -0.75 * np.exp(-(X * X + Y * Y + Z * Z) / 2)).astype(complex)
+    e = p.energies(psi)
+    assert e["norm"] == pytest.approx(1, rel=1e-12)
+    assert e["E"] == pytest.approx(3, rel=1e-10)
+    assert e["mu"] == pytest.approx(3, rel=1e-10)
+    assert e["E_kin"] == pytest.approx(1.5, rel=1e-10)
+
+
+def test_anisotropic_ground_state_energy():
+    """omega = (1, 1, 4): V = x^2 + y^2 + 16 z^2, ground state energy 1 + 1 + 4."""
+    p = orc.Problem(64, 16.0, "imaginary", 1e-3, 0.0, static="x**2 + y**2 + 16*z**2")
+    X, Y, Z = p.x[:, None, None], p.x[None, :, None], p.x[None, None, :]
+    psi = np.exp(-(X * X + Y * Y) / 2 - 2 * Z * Z).astype(complex)
+    psi = orc.normalise(psi, p.x)
+    assert p.energies(psi)["E"] == pytest.approx(6, rel=1e-9)
+
+
+def test_plane_wave_mu_and_energy():
+    """e^{ik.x}/L^{3/2} in V0: mu = k^2 + V0 + beta/L^3, E = k^2 + V0 + beta/(2 L^3)."""
+    length, beta, v0 = TWO_PI, 7.0, 0.5
+    p = orc.Problem(16, length, "real", 1e-3, beta, static="0.5")
+    phase = 2 * p.x[:, None, None] + 0 * p.x[None, :, None] + p.x[None, None, :]
+    psi = np.exp(1j * phase) / length ** 1.5
+    assert psi.shape == (16,) * 3
+    e = p.energies(psi)
+    assert e["mu"] == pytest.approx(5 + v0 + beta / length ** 3, rel=1e-12)
+    assert e["E"] == pytest.approx(5 + v0 + beta / (2 * length ** 3), rel=1e-12)
+
+
+def test_virial_identity_at_converged_ground_state():
+    """Harmonic trap: 2 E_kin - 2 E_pot + 3 E_int = 0 at the ground state; the
+    imaginary-time fixed point satisfies it up to O(dtau)."""
+
+    def residual(dt, psi, steps):
+        p = orc.Problem(32, 12.0, "imaginary", dt, 60.0, static="x**2 + y**2 + z**2")
+        psi = p.run(psi, steps)
+        e = p.energies(psi)
+        return abs(2 * e["E_kin"] - 2 * e["E_pot"] + 3 * e["E_int"]) / e["E"], psi
+
+    coarse, psi = residual(2e-3, orc.gaussian(orc.get_grids(32, 12.0)[0], 1.0), 1500)
+    fine, _ = residual(5e-4, psi, 2000)
+    assert coarse < 5e-3
+    assert fine < 0.35 * coarse
+
+
+def test_imaginary_time_lowers_energy_monotonically():
+    p = orc.Problem(16, 10.0, "imaginary", 5e-3, 20.0, static="x**2 + y**2 + z**2")
+    psi = orc.gaussian(p.x, 1.6)
+    energies = []
+    for _ in range(6):
+        psi = p.run(psi, 10)
+        energies.append(p.energies(psi)["E"])
+    assert all(b < a for a, b in zip(energies, energies[1:]))
