@@ -29,7 +29,8 @@ EXPORTS = (
     "gpe_psi_download", "gpe_psi_copy", "gpe_psi_gaussian", "gpe_potential_static",
     "gpe_potential_dynamic", "gpe_volume_fill", "gpe_volume_upload", "gpe_step",
     "gpe_norm_history", "gpe_norm", "gpe_normalise", "gpe_diagnostics", "gpe_sync",
-    "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum",
+    "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
+    "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
 )
 
 
@@ -85,6 +86,12 @@ def _declare(lib):
         "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
         "gpe_angular_momentum": (c_int, [vp, dp]),
+        "gpe_batch_create": (c_int, [vp, POINTER(vp), i32, POINTER(vp)]),
+        "gpe_batch_destroy": (c_int, [vp]),
+        "gpe_batch_step": (c_int, [vp, i64, i32]),
+        "gpe_batch_step_timed": (c_int, [vp, i64, dp, POINTER(i64)]),
+        "gpe_batch_launch": (c_int, [vp, i64, i32]),
+        "gpe_batch_wait": (c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -313,3 +320,42 @@ class Solve:
             result["counts"] = counts
             result["occupation"] = occ
         return result
+
+
+class Batch:
+    """Equal-shape solves advanced together, one launch per pass (C ABI
+    gpe_batch_*).  Members keep their own fields and parameters."""
+
+    def __init__(self, solves, ctx=None):
+        self.solves = list(solves)
+        self.ctx = ctx or self.solves[0].ctx
+        arr = (c_void_p * len(self.solves))(*[s.handle for s in self.solves])
+        handle = c_void_p()
+        check(lib().gpe_batch_create(self.ctx.handle, arr, len(self.solves), byref(handle)))
+        self.handle = handle
+
+    def step(self, nsteps, graph=True):
+        check(lib().gpe_batch_step(self.handle, int(nsteps), 1 if graph else 0))
+
+    def launch(self, nsteps, graph=True):
+        check(lib().gpe_batch_launch(self.handle, int(nsteps), 1 if graph else 0))
+
+    def wait(self):
+        check(lib().gpe_batch_wait(self.handle))
+
+    def step_timed(self, nsteps):
+        ms = np.zeros(len(PASS_KINDS))
+        cnt = np.zeros(len(PASS_KINDS), dtype=np.int64)
+        check(lib().gpe_batch_step_timed(self.handle, int(nsteps), dptr(ms), i64ptr(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().gpe_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

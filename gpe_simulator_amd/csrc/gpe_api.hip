@@ -409,18 +409,52 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag) {
   return GPE_OK;
 }
 
+// Pass sequence of `nsteps` steps without time-dependent programs, emitted
+// through pass_fn(kind, slot_a, slot_b, flags, hist_index).  Real time fuses
+// the second half-step of step i with the first of step i+1 (Z_MID); imaginary
+// time ends every step with the FP64 norm and defers the 1/sqrt(norm) scale to
+// the next load (F_PRESCALE).
+template <class P>
+int emit_static_steps(int kind, int64_t nsteps, bool prescale_first, P&& pass_fn) {
+  for (int64_t step = 0; step < nsteps; ++step) {
+    const int first_flags = (step == 0 ? prescale_first : true) ? F_PRESCALE : 0;
+    if (kind == GPE_REAL) {
+      if (step == 0) CHK(pass_fn(K_Z_FIRST, 0, 0, prescale_first ? F_PRESCALE : 0, 0));
+      CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
+      CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
+      CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
+      if (step == nsteps - 1)
+        CHK(pass_fn(K_Z_LAST, 0, 0, 0, 0));
+      else
+        CHK(pass_fn(K_Z_MID, 0, 0, 0, 0));
+    } else {
+      CHK(pass_fn(K_Z_FIRST, 0, 0, first_flags, 0));
+      CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
+      CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
+      CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
+      CHK(pass_fn(K_Z_LAST, 0, 0, F_NORM, (int)step));
+    }
+  }
+  return GPE_OK;
+}
+
 int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
   const bool dyn = any_dynamic(s);
+  if (!dyn) {
+    const bool pre = s->pending;
+    s->pending = false;
+    return emit_static_steps(GPE_REAL, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+      return pass(s, k, false, sa, sb, f, h);
+    });
+  }
   int64_t done = 0;
   while (done < nsteps) {
-    const int64_t cs = dyn ? std::min<int64_t>(s->chunk, nsteps - done) : nsteps;
-    if (dyn) {
-      const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
-      CHK(eval_dynamic(s, tmid + done, nslots, false));
-    }
+    const int64_t cs = std::min<int64_t>(s->chunk, nsteps - done);
+    const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
+    CHK(eval_dynamic(s, tmid + done, nslots, false));
     for (int64_t q = 0; q < cs; ++q) {
       const int64_t step = done + q;
-      const int sa = dyn ? (int)q : 0, sb = dyn ? (int)q + 1 : 0;
+      const int sa = (int)q, sb = (int)q + 1;
       if (step == 0) {
         CHK(pass(s, K_Z_FIRST, false, sa, 0, s->pending ? F_PRESCALE : 0));
         s->pending = false;
@@ -438,26 +472,26 @@ int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
   return GPE_OK;
 }
 
+int ensure_hist(gpe_solve* s, int64_t nsteps) {
+  if (nsteps <= s->hist_cap) return GPE_OK;
+  HIPCHK(hipStreamSynchronize(s->st));
+  dfree(s->hist);
+  s->hist = nullptr;
+  s->hist_cap = 0;
+  const long long cap = std::max<long long>(nsteps, 1024);
+  CHK(dzalloc(&s->hist, cap * sizeof(double)));
+  s->hist_cap = cap;
+  return refresh_desc(s);
+}
+
 int step_imag(gpe_solve* s, int64_t nsteps) {
-  if (nsteps > s->hist_cap) {
-    HIPCHK(hipStreamSynchronize(s->st));
-    dfree(s->hist);
-    s->hist = nullptr;
-    s->hist_cap = 0;
-    const long long cap = std::max<long long>(nsteps, 1024);
-    CHK(dzalloc(&s->hist, cap * sizeof(double)));
-    s->hist_cap = cap;
-    CHK(refresh_desc(s));
-  }
+  CHK(ensure_hist(s, nsteps));
   CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
-  for (int64_t step = 0; step < nsteps; ++step) {
-    CHK(pass(s, K_Z_FIRST, false, 0, 0, s->pending ? F_PRESCALE : 0));
-    CHK(pass(s, K_Y_FWD, false, 0, 0, 0));
-    CHK(pass(s, K_X_KIN, false, 0, 0, 0));
-    CHK(pass(s, K_Y_INV, false, 0, 0, 0));
-    CHK(pass(s, K_Z_LAST, false, 0, 0, F_NORM, (int)step));
-    s->pending = true;
-  }
+  const bool pre = s->pending;
+  CHK(emit_static_steps(GPE_IMAG, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+    return pass(s, k, false, sa, sb, f, h);
+  }));
+  s->pending = true;
   s->hist_len = nsteps;
   return GPE_OK;
 }
@@ -910,6 +944,195 @@ int gpe_angular_momentum(gpe_solve* s, double* out) {
   HIPCHK(hipStreamSynchronize(s->st));
   *out = (h[0] - h[1]) * s->dx3 / double(s->n);
   return GPE_OK;
+}
+
+// ------------------------------------------------------------- batches --
+struct gpe_batch {
+  gpe_ctx* ctx = nullptr;
+  std::vector<gpe_solve*> solves;
+  int n = 0, prec = 0, kind = 0;
+  hipStream_t st = nullptr;
+  void* desc = nullptr;                 // device array of per-solve descriptors
+  PassLauncher launch = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_steps = 0;
+  bool timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double ms[kPassKinds] = {0};
+  long long cnt[kPassKinds] = {0};
+};
+
+int gpe_batch_destroy(gpe_batch* b) {
+  if (!b) return GPE_OK;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  dfree(b->desc);
+  for (auto e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+  return GPE_OK;
+}
+
+int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_batch** out) {
+  if (!ctx || !solves || count <= 0 || !out) return fail(GPE_ERR_INVALID, "bad batch arguments");
+  for (int i = 0; i < count; ++i) {
+    gpe_solve* s = solves[i];
+    if (!s || s->ctx != ctx) return fail(GPE_ERR_INVALID, "batch solve missing or on another context");
+    if (s->n != solves[0]->n || s->prec != solves[0]->prec || s->kind != solves[0]->kind)
+      return fail(GPE_ERR_INVALID, "a batch needs solves of one side, precision and kind");
+    if (any_dynamic(s)) return fail(GPE_ERR_INVALID, "batched solves cannot carry time-dependent programs");
+    for (int j = 0; j < i; ++j)
+      if (solves[j] == s) return fail(GPE_ERR_INVALID, "a solve appears twice in the batch");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  gpe_batch* b = new gpe_batch();
+  b->ctx = ctx;
+  b->solves.assign(solves, solves + count);
+  b->n = solves[0]->n;
+  b->prec = solves[0]->prec;
+  b->kind = solves[0]->kind;
+  b->launch = solves[0]->launch;
+  auto bail = [&](int code) {
+    gpe_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
+    return bail(fail(GPE_ERR_HIP, "batch stream/event creation failed"));
+  const size_t dsz = b->prec == GPE_C64 ? sizeof(SolveDev<float>) : sizeof(SolveDev<double>);
+  if (int r = dmalloc(&b->desc, dsz * count)) return bail(r);
+  *out = b;
+  return GPE_OK;
+}
+
+namespace {
+
+// Bring every member to the state a batched step sequence expects and copy
+// the descriptors into the batch's device array.
+int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
+  const size_t dsz = b->prec == GPE_C64 ? sizeof(SolveDev<float>) : sizeof(SolveDev<double>);
+  *prescale_first = false;
+  for (gpe_solve* s : b->solves) {
+    if (b->kind == GPE_IMAG) {
+      CHK(ensure_hist(s, nsteps));
+      CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
+      if (!s->pending) {
+        // a unit pending scale lets every member take the prescaled path
+        const double one = 1.0;
+        HIPCHK(hipMemcpyAsync(s->scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, s->st));
+        s->pending = true;
+      }
+      *prescale_first = true;
+    } else {
+      CHK(materialize(s));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  std::vector<char> host(dsz * b->solves.size());
+  for (size_t i = 0; i < b->solves.size(); ++i) std::memcpy(host.data() + i * dsz, b->solves[i]->h_desc_step.data(), dsz);
+  HIPCHK(hipMemcpyAsync(b->desc, host.data(), host.size(), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return GPE_OK;
+}
+
+int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
+  PassArgsRaw a{b->desc, sa, sb, flags, hist};
+  if (b->timing) HIPCHK(hipEventRecord(b->ev[0], b->st));
+  hipError_t e = b->launch(kind, &a, b->n, (int)b->solves.size(), b->st);
+  if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
+  if (b->timing) {
+    HIPCHK(hipEventRecord(b->ev[1], b->st));
+    HIPCHK(hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->ms[kind] += ms;
+    b->cnt[kind] += 1;
+  }
+  return GPE_OK;
+}
+
+void batch_finish(gpe_batch* b, int64_t nsteps) {
+  for (gpe_solve* s : b->solves) {
+    if (b->kind == GPE_IMAG) {
+      s->pending = true;
+      s->hist_len = nsteps;
+    } else {
+      s->pending = false;
+    }
+  }
+}
+
+}  // namespace
+
+int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
+  if (!b) return fail(GPE_ERR_INVALID, "null batch");
+  if (nsteps < 0) return fail(GPE_ERR_INVALID, "nsteps must be >= 0");
+  if (nsteps == 0) return GPE_OK;
+  HIPCHK(hipSetDevice(b->ctx->device));
+  bool pre = false;
+  CHK(batch_prepare(b, nsteps, &pre));
+  auto emit = [&]() {
+    return emit_static_steps(b->kind, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+      return batch_pass(b, k, sa, sb, f, h);
+    });
+  };
+  if (!use_graph || b->timing) {
+    CHK(emit());
+  } else {
+    if (!b->exec || b->graph_steps != nsteps) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      const int r = emit();
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->st, &g);
+      if (r != GPE_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      b->graph = g;
+      HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_steps = nsteps;
+    }
+    HIPCHK(hipGraphLaunch(b->exec, b->st));
+  }
+  batch_finish(b, nsteps);
+  return GPE_OK;
+}
+
+int gpe_batch_wait(gpe_batch* b) {
+  if (!b) return fail(GPE_ERR_INVALID, "null batch");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return GPE_OK;
+}
+
+int gpe_batch_step(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
+  CHK(gpe_batch_launch(b, nsteps, use_graph));
+  return gpe_batch_wait(b);
+}
+
+int gpe_batch_step_timed(gpe_batch* b, int64_t nsteps, double* ms_per_kind, int64_t* launches_per_kind) {
+  if (!b || !ms_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
+  for (int k = 0; k < kPassKinds; ++k) {
+    b->ms[k] = 0;
+    b->cnt[k] = 0;
+  }
+  b->timing = true;
+  const int r = gpe_batch_step(b, nsteps, 0);
+  b->timing = false;
+  for (int k = 0; k < kPassKinds; ++k) {
+    ms_per_kind[k] = b->ms[k];
+    launches_per_kind[k] = b->cnt[k];
+  }
+  return r;
 }
 
 int gpe_sync(gpe_solve* s) {

@@ -155,11 +155,11 @@ __global__ void k_add_f64(double* out, const double* in, long long size) {
 // ------------------------------------------------------------ fields --
 template <typename R>
 __global__ void k_scale(cpx<R>* psi, long long count, const double* scal) {
-  const R s = R(scal[1]);
+  const double s = scal[1];
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < count;
        p += (long long)gridDim.x * blockDim.x) {
     const cpx<R> v = psi[p];
-    psi[p] = cpx<R>{v.x * s, v.y * s};
+    psi[p] = cpx<R>{R(double(v.x) * s), R(double(v.y) * s)};
   }
 }
 

@@ -309,7 +309,9 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
   const bool has_vim = s.vim != nullptr;
   const bool prescale = (a.flags & F_PRESCALE) != 0;
-  const R sc = prescale ? R(s.scal[1]) : R(1);
+  // the pending 1/sqrt(norm) is applied in FP64 so the renormalised state
+  // has unit norm to rounding, not to the float representation of the scale
+  const double sc = prescale ? s.scal[1] : 1.0;
 
   // Reductions accumulated over all tiles of the block.
   double acc[3] = {0.0, 0.0, 0.0};
@@ -377,7 +379,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         TF::template fft<+1>(v, lp, t, twp, n);
       } else if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
       }
       const long long rowoff = g.base;
       const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
       }
       const long long rowoff = g.base;
       const double pxa = double(gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj)));
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
       if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cpx<R>{v[m].x * sc, v[m].y * sc};
+        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
       }
       TF::template fft<-1>(v, lp, t, twp, n);
       const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);

@@ -1,0 +1,234 @@
+"""Continuous-batching solve server.
+
+Requests (imaginary-time ground states to a mu tolerance, or fixed-length
+real-time runs) arrive over time.  The scheduler keeps every admitted field
+resident in HBM and, once per tick:
+
+  1. admits waiting requests while free HBM (gpe_mem_info) minus a reserve
+     covers the request's resident + diagnostics footprint (gpe_solve_footprint);
+  2. groups the active solves by (grid side, precision, kind) and enqueues
+     `check_every` steps for every group as one batch (one launch per pass
+     for all members, replayed from a hipGraph), each group on its own HIP
+     stream so groups overlap on the device, then waits for all groups;
+  3. reads each member's per-step norms, decides convergence (imaginary time:
+     |dmu| < tol on the norm-decay estimate, with complex64 members moving to
+     a complex128 field below the complex64 noise floor) or completion (real
+     time: the requested step count), computes the FP64 energies of finished
+     solves, returns them and frees their memory.
+
+A solve's arithmetic does not depend on its batch-mates: every member has its
+own field, tables and reductions, and every tick advances it by the same
+number of steps, so a result is the same alone and in the mix.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .engine import make_solve
+from .grids import get_grids
+from .sweep import mu_from_norms
+from .units import harmonic_expression
+
+
+@dataclass
+class Request:
+    id: int
+    arrival: float             # seconds after the stream starts
+    n: int
+    kind: str                  # "imaginary" or "real"
+    precision: str             # "c64" or "c128"
+    beta: float
+    length: float = 20.0
+    static: str = "x**2 + y**2 + z**2"
+    dt: float = 5e-4
+    sigma: float = 1.0
+    tol: float = 2e-9          # imaginary time: |dmu| tolerance (reference units)
+    steps: int = 2000          # real time: steps to run
+    max_steps: int = 400_000
+
+
+@dataclass
+class Outcome:
+    id: int
+    n: int
+    kind: str
+    precision: str
+    latency: float             # arrival -> result on the host, seconds
+    queued: float              # arrival -> admission, seconds
+    steps: int
+    mu: float
+    energy: float
+    norm: float
+    dmu: float
+    converged: bool
+
+
+@dataclass
+class _Slot:
+    req: Request
+    solve: object
+    precision: str
+    admitted: float
+    steps: int = 0
+    dmu: float = float("inf")
+
+
+def _footprint(req, precision):
+    resident, diag = _lib.footprint(req.n, _lib.C64 if precision == "c64" else _lib.C128,
+                                    _lib.IMAG if req.kind == "imaginary" else _lib.REAL)
+    return resident + diag
+
+
+class Server:
+    def __init__(self, ctx=None, check_every=50, switch_tol=2e-6, reserve_bytes=2 << 30, graph=True):
+        self.ctx = ctx or _lib.default_context()
+        self.check_every = int(check_every)
+        self.switch_tol = switch_tol
+        self.reserve = reserve_bytes
+        self.graph = graph
+        self._batches = {}
+        self.grid_point_steps = 0
+        self.max_active = 0
+
+    # -- solve lifecycle
+    def _new_solve(self, req, precision):
+        x, k = get_grids(req.n, req.length)
+        return make_solve(req.n, x, k, req.kind, req.dt, req.beta, static=req.static, precision=precision,
+                          ctx=self.ctx)
+
+    def _admit(self, req, now):
+        solve = self._new_solve(req, req.precision)
+        solve.gaussian(req.sigma)
+        return _Slot(req=req, solve=solve, precision=req.precision, admitted=now)
+
+    def _batch_for(self, slots):
+        key = tuple(id(s.solve) for s in slots)
+        batch = self._batches.get(key)
+        if batch is None:
+            batch = _lib.Batch([s.solve for s in slots], ctx=self.ctx)
+            self._batches[key] = batch
+        return batch
+
+    def _drop_batches(self, solve):
+        for key in [k for k in self._batches if id(solve) in k]:
+            self._batches.pop(key).close()
+
+    # -- main loop
+    def run(self, requests, realtime=True, log=None):
+        """Serve `requests` (arrival-ordered) and return (outcomes, stats)."""
+        start = time.perf_counter()
+        incoming = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting, active, outcomes = deque(), [], []
+        clock = (lambda: time.perf_counter() - start) if realtime else None
+        virtual = 0.0
+        while incoming or waiting or active:
+            now = clock() if realtime else virtual
+            while incoming and incoming[0].arrival <= now:
+                waiting.append(incoming.popleft())
+            if not active and not waiting:
+                if realtime:
+                    time.sleep(max(0.0, incoming[0].arrival - now))
+                else:
+                    virtual = incoming[0].arrival
+                continue
+            # admission by free device memory
+            free, _total = self.ctx.mem_info()
+            for req in list(waiting):
+                need = _footprint(req, req.precision)
+                if req.precision == "c64" and req.kind == "imaginary":
+                    need += _footprint(req, "c128")   # room for the complex128 polish
+                if free - self.reserve >= need or not active:
+                    waiting.remove(req)
+                    active.append(self._admit(req, now))
+                    free -= need
+            self.max_active = max(self.max_active, len(active))
+            # one tick: every group enqueued, then all awaited
+            groups = {}
+            for slot in active:
+                groups.setdefault((slot.req.n, slot.precision, slot.req.kind), []).append(slot)
+            launched = []
+            for members in groups.values():
+                batch = self._batch_for(members)
+                batch.launch(self.check_every, graph=self.graph)
+                launched.append(batch)
+            for batch in launched:
+                batch.wait()
+            if not realtime:
+                virtual += 0.0
+            # progress, convergence and completion
+            finished = []
+            for slot in active:
+                slot.steps += self.check_every
+                self.grid_point_steps += slot.req.n ** 3 * self.check_every
+                req = slot.req
+                if req.kind == "imaginary":
+                    mu = mu_from_norms(slot.solve.norm_history(self.check_every), req.dt)
+                    slot.dmu = float(abs(mu[-1] - mu[-2]))
+                    if slot.precision == "c64" and slot.dmu < self.switch_tol:
+                        self._promote(slot)
+                    elif slot.precision == "c128" and slot.dmu < req.tol:
+                        finished.append(slot)
+                    elif slot.steps >= req.max_steps:
+                        finished.append(slot)
+                elif slot.steps >= req.steps:
+                    finished.append(slot)
+            for slot in finished:
+                outcomes.append(self._complete(slot, clock() if realtime else virtual))
+                active.remove(slot)
+                if log:
+                    log(outcomes[-1])
+        elapsed = time.perf_counter() - start
+        lat = np.array([o.latency for o in outcomes]) * 1e3
+        stats = dict(requests=len(outcomes), seconds=elapsed,
+                     grid_point_steps_per_s=self.grid_point_steps / elapsed if elapsed > 0 else 0.0,
+                     p50_ms=float(np.percentile(lat, 50)) if len(lat) else None,
+                     p99_ms=float(np.percentile(lat, 99)) if len(lat) else None,
+                     max_active=self.max_active)
+        for batch in self._batches.values():
+            batch.close()
+        self._batches.clear()
+        return outcomes, stats
+
+    def _promote(self, slot):
+        """Move a complex64 imaginary-time member to a complex128 field."""
+        fresh = self._new_solve(slot.req, "c128")
+        fresh.copy_from(slot.solve)
+        self._drop_batches(slot.solve)
+        slot.solve.close()
+        slot.solve, slot.precision = fresh, "c128"
+
+    def _complete(self, slot, now):
+        req = slot.req
+        diag = slot.solve.diagnostics(req.steps * req.dt if req.kind == "real" else 0.0, spectrum=False)
+        self._drop_batches(slot.solve)
+        slot.solve.close()
+        converged = req.kind == "real" or slot.dmu < req.tol
+        return Outcome(id=req.id, n=req.n, kind=req.kind, precision=req.precision, latency=now - req.arrival,
+                       queued=slot.admitted - req.arrival, steps=slot.steps, mu=diag["mu"], energy=diag["E"],
+                       norm=diag["norm"], dmu=slot.dmu, converged=converged)
+
+
+def request_stream(count, rate_per_s=2.0, seed=0, sides=(64, 96, 128, 192, 256),
+                   side_p=(0.3, 0.25, 0.2, 0.15, 0.1), imag_fraction=0.7, c64_fraction=0.8,
+                   g_range=(50.0, 2000.0), length=20.0, dt_osc=1e-3, tol_osc=1e-9, real_steps=2000):
+    """Seeded Poisson stream of mixed requests (oscillator-unit parameters
+    mapped to reference units: beta = 2g, dtau = dtau'/2, tol doubled)."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate_per_s)
+        n = int(rng.choice(sides, p=side_p))
+        kind = "imaginary" if rng.random() < imag_fraction else "real"
+        precision = "c64" if rng.random() < c64_fraction else "c128"
+        g = rng.uniform(*g_range)
+        out.append(Request(id=i, arrival=t, n=n, kind=kind, precision=precision, beta=2 * g, length=length,
+                           static=harmonic_expression(1, 1, 1), dt=dt_osc / 2, sigma=1.0, tol=2 * tol_osc,
+                           steps=real_steps))
+    return out

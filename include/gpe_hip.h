@@ -25,6 +25,7 @@ extern "C" {
 
 typedef struct gpe_ctx gpe_ctx;
 typedef struct gpe_solve gpe_solve;
+typedef struct gpe_batch gpe_batch;
 
 enum {
   GPE_OK = 0,
@@ -144,6 +145,26 @@ int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts
 /* Lz = Re int psi^* (x p_y - y p_x) psi dV with spectral derivatives, FP64.
  * Reference: get_L (gpe/helper.py:143-156). */
 int gpe_angular_momentum(gpe_solve* solve, double* out);
+
+/* Batches: equal-shape solves (same side, precision and kind, no
+ * time-dependent program) advanced together, every pass one launch over all
+ * members (the member index is the third grid dimension).  Each member keeps
+ * its own psi, potential, beta, dt and reductions, so its result is the same
+ * as when it runs alone.  With use_graph != 0 the nsteps-step launch sequence
+ * is captured once in a hipGraph and replayed.  The batch borrows its
+ * members: do not step a member alone while its batch is stepping.  No
+ * reference counterpart (the reference runs one stage at a time,
+ * gpe/main.py:30-38). */
+int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_batch** out);
+int gpe_batch_destroy(gpe_batch* batch);
+int gpe_batch_step(gpe_batch* batch, int64_t nsteps, int32_t use_graph);
+/* Asynchronous form: enqueue the steps on the batch's own stream and return;
+ * gpe_batch_wait blocks until they are done.  Batches of different shapes
+ * launched back to back overlap on the device.  Members must not be used
+ * until the wait returns. */
+int gpe_batch_launch(gpe_batch* batch, int64_t nsteps, int32_t use_graph);
+int gpe_batch_wait(gpe_batch* batch);
+int gpe_batch_step_timed(gpe_batch* batch, int64_t nsteps, double* ms_per_kind, int64_t* launches_per_kind);
 
 /* Wait for all work of the solve. */
 int gpe_sync(gpe_solve* solve);
