@@ -8,6 +8,11 @@ histogram every 500 steps.  A step is one Strang step over the whole grid;
 the state is a seeded synthetic Thomas-Fermi profile plus noise, resident in
 HBM before timing starts.
 
+Beside it, the same line reports: the complex128 run of the same workload,
+per-kernel HIP-event timings, the oracle on the host as the CPU baseline, a
+short seeded serving stream (p50/p99 time-to-result) and the 64-solve 96^3
+batched ground-state sweep against sequential complex128 solves.
+
 Prints ONE JSON line.  `value` = grid-point-steps/s of the complex64 run.
 """
 
@@ -15,7 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
-import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -34,6 +39,7 @@ BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
 NK_INTERVAL = 500
 HBM_PEAK_GBS = 8000.0
 PASSES_PER_STEP = 4          # steady-state psi sweeps per real-time step (fused z pass)
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_stirring256_c64.txt"
 
 
 def log(msg):
@@ -81,6 +87,47 @@ def kernel_timing(solve, steps):
     return solve.step_timed(steps, tm)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE
+    + WRITE_SIZE, each scaled by the copy-kernel calibration), else None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        m = re.match(r"\s*traffic\s+(\w+)\s+([0-9.eE+]+)\s*$", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(2))
+    return None
+
+
+def serving(count, seed=0):
+    """Seeded Poisson stream of mixed requests (sides 64..256, 70% ground
+    states to |dmu| < 2e-9, 30% 2000-step real-time runs, 80% complex64)."""
+    from gpe_simulator_amd.serve import Server, request_stream
+    reqs = request_stream(count, rate_per_s=4.0, seed=seed)
+    outcomes, stats = Server(check_every=50).run(reqs)
+    stats["converged"] = sum(o.converged for o in outcomes)
+    stats["stream"] = f"{count} requests, Poisson 4/s, seed {seed}"
+    return stats
+
+
+def sweep(count, sequential_sample=4, seed=0):
+    """64 ground states at 96^3 (seeded g, omega_z, widths) batched, against
+    sequential complex128 solves of the first `sequential_sample` requests
+    (the sequential time is extrapolated to all requests)."""
+    from gpe_simulator_amd.sweep import config_sweep, solve_sequential, sweep_batched
+    reqs = config_sweep(count=count, seed=seed)
+    t0 = time.perf_counter()
+    res = sweep_batched(reqs)
+    batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    seq = [solve_sequential(r) for r in reqs[:sequential_sample]]
+    per_solve = (time.perf_counter() - t0) / sequential_sample
+    return {"requests": count, "grid": reqs[0].n, "batched_s": batched,
+            "sequential_s_est": per_solve * count, "speedup": per_solve * count / batched,
+            "converged": sum(r.converged for r in res),
+            "mu_rel_diff_vs_sequential": max(abs(a.mu - b.mu) / abs(b.mu) for a, b in zip(res, seq))}
+
+
 def cpu_baseline(psi0, samples):
     """The FP64 oracle (numpy.fft, single thread) on a bounded sample."""
     from oracle import gpe_oracle as orc
@@ -104,6 +151,8 @@ def main(argv=None):
     parser.add_argument("--cpu-samples", type=int, default=3)
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-c128", action="store_true")
+    parser.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving stream")
+    parser.add_argument("--sweep-count", type=int, default=64, help="0 skips the batched sweep")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
@@ -111,7 +160,7 @@ def main(argv=None):
         return 2
 
     from gpe_simulator_amd import _lib
-    from oracle.gpe_oracle import get_grids
+    from gpe_simulator_amd.grids import get_grids
     x, k = get_grids(N, LENGTH)
     log("building synthetic state")
     psi0 = synthetic_state(x)
@@ -152,6 +201,16 @@ def main(argv=None):
                "sample": f"{args.cpu_samples} Strang steps of the same 256^3 stirring workload "
                          f"(complex128 numpy.fft, 1 thread) after 1 warm-up step, {el:.1f} s"}
 
+    served = None
+    if args.serve_requests > 0:
+        log(f"serving stream: {args.serve_requests} requests")
+        served = serving(args.serve_requests)
+
+    swept = None
+    if args.sweep_count > 0:
+        log(f"batched sweep: {args.sweep_count} ground states at 96^3")
+        swept = sweep(args.sweep_count)
+
     step_bytes = PASSES_PER_STEP * 2 * 8
     line = {
         "metric": "3D SSFM grid-pt-steps/s at 256^3 c64 (% HBM roofline) vs c128; p50/p99 solve ms",
@@ -169,15 +228,18 @@ def main(argv=None):
         "config": {"workload": "stirring run 256^3, L=40, beta=4000, dt=2.5e-4, nk every 500 steps",
                    "grid": N, "precision": "complex64 field, FP64 reductions/time"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
+                     "algorithmic_bytes": bytes_per_launch, "avg_us": dom_ms * 1e3},
         "step_roofline_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
         "stream_copy_gbs": copy_gbs,
         "per_kernel": per_kernel,
         "mu_c64": diag64["mu"],
         "c128": result_c128,
         "cpu_baseline": cpu,
-        "p50_solve_ms": None,
-        "p99_solve_ms": None,
+        "p50_solve_ms": served["p50_ms"] if served else None,
+        "p99_solve_ms": served["p99_ms"] if served else None,
+        "serving": served,
+        "sweep": swept,
     }
     print(json.dumps(line), flush=True)
     return 0

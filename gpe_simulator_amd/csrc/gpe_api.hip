@@ -160,10 +160,15 @@ int dmalloc(void** p, size_t bytes) {
   return GPE_OK;
 }
 
+// Zero-filled allocation, cleared on the owning stream `st` and complete on
+// return.  (hipMemset would run on the null stream, which the non-blocking
+// solve and batch streams do not wait for: the clear could land after a
+// kernel on another queue had already written the buffer.)
 template <typename T>
-int dzalloc(T** p, size_t bytes) {
+int dzalloc(T** p, size_t bytes, hipStream_t st) {
   CHK(dmalloc((void**)p, bytes));
-  HIPCHK(hipMemset(*p, 0, bytes));
+  HIPCHK(hipMemsetAsync(*p, 0, bytes, st));
+  HIPCHK(hipStreamSynchronize(st));
   return GPE_OK;
 }
 
@@ -239,7 +244,7 @@ int ensure_red(gpe_solve* s, long long need) {
   dfree(s->red);
   s->red = nullptr;
   s->red_cap = 0;
-  CHK(dzalloc(&s->red, need * sizeof(double)));
+  CHK(dzalloc(&s->red, need * sizeof(double), s->st));
   s->red_cap = need;
   return refresh_desc(s);
 }
@@ -336,8 +341,8 @@ int upload_program(DevProgram& p, const int32_t* code, int32_t ncode, const doub
 
 int ensure_volume(gpe_solve* s, int comp) {
   if (s->m_vol[comp]) return GPE_OK;
-  CHK(dzalloc(&s->m_vol[comp], s->n3 * sizeof(double)));
-  CHK(dzalloc(&s->w_vol[comp], s->n3 * s->rsz));
+  CHK(dzalloc(&s->m_vol[comp], s->n3 * sizeof(double), s->st));
+  CHK(dzalloc(&s->w_vol[comp], s->n3 * s->rsz, s->st));
   return refresh_desc(s);
 }
 
@@ -365,12 +370,12 @@ int layout_dynamic(gpe_solve* s) {
     s->d_tab[g] = s->g_tab[g] = nullptr;
     if (!s->dyn[g].active) continue;
     const long long sz = target_size(s, g == 2 ? GPE_TGT_VOL : g);
-    CHK(dzalloc(&s->d_tab[g], (size_t)(chunk + 1) * sz * s->rsz));
-    CHK(dzalloc(&s->g_tab[g], (size_t)sz * s->rsz));
+    CHK(dzalloc(&s->d_tab[g], (size_t)(chunk + 1) * sz * s->rsz, s->st));
+    CHK(dzalloc(&s->g_tab[g], (size_t)sz * s->rsz, s->st));
   }
   dfree(s->d_tlist);
   s->d_tlist = nullptr;
-  CHK(dzalloc(&s->d_tlist, (size_t)(chunk + 2) * sizeof(double)));
+  CHK(dzalloc(&s->d_tlist, (size_t)(chunk + 2) * sizeof(double), s->st));
   for (int f = 0; f < 2; ++f) {
     if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
     s->h_tlist[f] = nullptr;
@@ -479,7 +484,7 @@ int ensure_hist(gpe_solve* s, int64_t nsteps) {
   s->hist = nullptr;
   s->hist_cap = 0;
   const long long cap = std::max<long long>(nsteps, 1024);
-  CHK(dzalloc(&s->hist, cap * sizeof(double)));
+  CHK(dzalloc(&s->hist, cap * sizeof(double), s->st));
   s->hist_cap = cap;
   return refresh_desc(s);
 }
@@ -518,18 +523,18 @@ int norm_any(gpe_solve* s) { return s->prec == GPE_C64 ? norm_t<float>(s) : norm
 
 int ensure_diag(gpe_solve* s) {
   if (s->work) return GPE_OK;
-  CHK(dzalloc(&s->work, s->n3 * s->esz));
+  CHK(dzalloc(&s->work, s->n3 * s->esz, s->st));
   const long long bx = s->blocks(K_X_DIAG, s->n);
-  CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double)));
-  CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int)));
-  CHK(dzalloc(&s->d_out, 16 * sizeof(double)));
-  CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double)));
-  CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long)));
+  CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double), s->st));
+  CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int), s->st));
+  CHK(dzalloc(&s->d_out, 16 * sizeof(double), s->st));
+  CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double), s->st));
+  CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long), s->st));
   const long long need = std::max(s->blocks(K_Z_DIAG, s->n) * 3, bx * 2);
   if (need > s->red_cap) {
     dfree(s->red);
     s->red = nullptr;
-    CHK(dzalloc(&s->red, need * sizeof(double)));
+    CHK(dzalloc(&s->red, need * sizeof(double), s->st));
     s->red_cap = need;
   }
   return refresh_desc(s);
@@ -629,13 +634,13 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   }
   int r;
   const int n = s->n;
-  if ((r = dzalloc(&s->psi, s->n3 * s->esz)) || (r = dzalloc(&s->tw, n * 16)) ||
-      (r = dzalloc(&s->kin[0], n * 16)) || (r = dzalloc(&s->kin[1], n * 16)) ||
-      (r = dzalloc(&s->kin[2], n * 16)) || (r = dzalloc(&s->xa, n * sizeof(double))) ||
-      (r = dzalloc(&s->ka, n * sizeof(double))) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double))) ||
-      (r = dzalloc(&s->m_z, n * sizeof(double))) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz)) ||
-      (r = dzalloc(&s->w_z, n * s->rsz)) || (r = dzalloc(&s->counter, 64)) ||
-      (r = dzalloc(&s->scal, 8 * sizeof(double))) ||
+  if ((r = dzalloc(&s->psi, s->n3 * s->esz, s->st)) || (r = dzalloc(&s->tw, n * 16, s->st)) ||
+      (r = dzalloc(&s->kin[0], n * 16, s->st)) || (r = dzalloc(&s->kin[1], n * 16, s->st)) ||
+      (r = dzalloc(&s->kin[2], n * 16, s->st)) || (r = dzalloc(&s->xa, n * sizeof(double), s->st)) ||
+      (r = dzalloc(&s->ka, n * sizeof(double), s->st)) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double), s->st)) ||
+      (r = dzalloc(&s->m_z, n * sizeof(double), s->st)) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz, s->st)) ||
+      (r = dzalloc(&s->w_z, n * s->rsz, s->st)) || (r = dzalloc(&s->counter, 64, s->st)) ||
+      (r = dzalloc(&s->scal, 8 * sizeof(double), s->st)) ||
       (r = dmalloc(&s->desc_step, sizeof(SolveDev<double>))) ||
       (r = dmalloc(&s->desc_diag, sizeof(SolveDev<double>))))
     return bail(r);
@@ -826,7 +831,9 @@ int gpe_volume_upload(gpe_solve* s, int32_t target, const double* host, int64_t 
   const int comp = target == GPE_TGT_VOL ? 0 : 1;
   CHK(ensure_volume(s, comp));
   if (!accumulate) {
-    HIPCHK(hipMemcpy(s->m_vol[comp] + offset, host, count * sizeof(double), hipMemcpyHostToDevice));
+    // on the solve's stream: earlier kernels may still read the table
+    HIPCHK(hipMemcpyAsync(s->m_vol[comp] + offset, host, count * sizeof(double), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
   } else {
     CHK(ensure_stage(s));
     const int64_t per = kStageElems * 2;   // doubles per staging copy
@@ -932,7 +939,7 @@ int gpe_angular_momentum(gpe_solve* s, double* out) {
   CHK(set_device(s));
   const long long bx = s->blocks(K_X_LZ, s->n), by = s->blocks(K_Y_LZ, s->n);
   CHK(ensure_red(s, std::max(bx, by)));
-  if (!s->d_out) CHK(dzalloc(&s->d_out, 16 * sizeof(double)));
+  if (!s->d_out) CHK(dzalloc(&s->d_out, 16 * sizeof(double), s->st));
   const int pre = s->pending ? F_PRESCALE : 0;
   CHK(pass(s, K_Y_LZ, true, 0, 0, pre));
   launch_finish_sums(s->red, by, 1, s->d_out + 8, s->st);

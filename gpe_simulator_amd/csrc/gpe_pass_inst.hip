@@ -51,7 +51,12 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
   switch (kind) {
     case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a); break;
     case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a); break;
-    case K_Z_LAST: go<2, K_Z_LAST>(n, batch, st, a); break;
+    case K_Z_LAST:
+      go<2, K_Z_LAST>(n, batch, st, a);
+      if (a.flags & F_NORM)
+        hipLaunchKernelGGL((k_norm_finish<Real>), dim3((unsigned)batch), dim3(256), 0, st, a,
+                           (int)tile_count<TF, 2>(n));
+      break;
     case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a); break;
     case K_Z_FWD: go<2, K_Z_FWD>(n, batch, st, a); break;
     case K_Y_FWD: go<1, K_Y_FWD>(n, batch, st, a); break;

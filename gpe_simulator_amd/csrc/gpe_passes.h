@@ -484,32 +484,14 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const unsigned bid = blockIdx.x;
   if constexpr (KIND == K_Z_LAST) {
     if (a.flags & F_NORM) {
-      // FP64 block partial, then the last block to finish sums all partials
-      // in a fixed order and publishes norm and 1/sqrt(norm).
+      // FP64 block partial only; k_norm_finish (launched right after on the
+      // same stream) sums the partials.  An agent-scope fence + ticket here
+      // would write back the XCD's L2 once per block, which costs more than
+      // the tile itself on small grids.
       __shared__ double red_scratch[16];
-      __shared__ int is_last;
       double part[1] = {acc[0]};
       block_sum<1>(part, red_scratch);
-      if (threadIdx.x == 0) {
-        s.red[bid] = part[0];
-        __threadfence();
-        const unsigned ticket = atomicAdd(s.counter, 1u);
-        is_last = (ticket == nblk - 1);
-      }
-      __syncthreads();
-      if (is_last) {
-        __threadfence();
-        double tot[1] = {0.0};
-        for (unsigned q = threadIdx.x; q < nblk; q += blockDim.x) tot[0] += coherent_load(s.red + q);
-        block_sum<1>(tot, red_scratch);
-        if (threadIdx.x == 0) {
-          const double norm = s.dx3 * tot[0];
-          s.scal[0] = norm;
-          s.scal[1] = 1.0 / sqrt(norm);
-          if (s.hist) s.hist[a.hist_index] = norm;
-          *s.counter = 0u;
-        }
-      }
+      if (threadIdx.x == 0) s.red[bid] = part[0];
     }
   } else if constexpr (KIND == K_Z_DIAG) {
     __shared__ double red_scratch[48];
@@ -537,6 +519,23 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       s.nkocc[(long long)bid * s.nbins + q] = hocc[q];
       s.nkcnt[(long long)bid * s.nbins + q] = hcnt[q];
     }
+  }
+}
+
+// Sums the Z_LAST norm partials of one solve per block (blockIdx.x = member),
+// in a fixed order, and publishes norm, 1/sqrt(norm) and the per-step norm.
+template <class R>
+__global__ void __launch_bounds__(256) k_norm_finish(PassArgs<R> a, int nblk) {
+  const SolveDev<R>& s = a.s[blockIdx.x];
+  __shared__ double red_scratch[16];
+  double tot[1] = {0.0};
+  for (int q = threadIdx.x; q < nblk; q += blockDim.x) tot[0] += s.red[q];
+  block_sum<1>(tot, red_scratch);
+  if (threadIdx.x == 0) {
+    const double norm = s.dx3 * tot[0];
+    s.scal[0] = norm;
+    s.scal[1] = 1.0 / sqrt(norm);
+    if (s.hist) s.hist[a.hist_index] = norm;
   }
 }
 

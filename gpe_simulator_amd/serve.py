@@ -12,7 +12,7 @@ resident in HBM and, once per tick:
      stream so groups overlap on the device, then waits for all groups;
   3. reads each member's per-step norms, decides convergence (imaginary time:
      |dmu| < tol on the norm-decay estimate, with complex64 members moving to
-     a complex128 field below the complex64 noise floor) or completion (real
+     a complex128 field when their Switch says so) or completion (real
      time: the requested step count), computes the FP64 energies of finished
      solves, returns them and frees their memory.
 
@@ -33,7 +33,7 @@ import numpy as np
 from . import _lib
 from .engine import make_solve
 from .grids import get_grids
-from .sweep import mu_from_norms
+from .sweep import Switch, mu_from_norms
 from .units import harmonic_expression
 
 
@@ -78,6 +78,7 @@ class _Slot:
     admitted: float
     steps: int = 0
     dmu: float = float("inf")
+    switch: Switch = None
 
 
 def _footprint(req, precision):
@@ -87,10 +88,9 @@ def _footprint(req, precision):
 
 
 class Server:
-    def __init__(self, ctx=None, check_every=50, switch_tol=2e-6, reserve_bytes=2 << 30, graph=True):
+    def __init__(self, ctx=None, check_every=50, reserve_bytes=2 << 30, graph=True):
         self.ctx = ctx or _lib.default_context()
         self.check_every = int(check_every)
-        self.switch_tol = switch_tol
         self.reserve = reserve_bytes
         self.graph = graph
         self._batches = {}
@@ -106,7 +106,7 @@ class Server:
     def _admit(self, req, now):
         solve = self._new_solve(req, req.precision)
         solve.gaussian(req.sigma)
-        return _Slot(req=req, solve=solve, precision=req.precision, admitted=now)
+        return _Slot(req=req, solve=solve, precision=req.precision, admitted=now, switch=Switch(req.tol))
 
     def _batch_for(self, slots):
         key = tuple(id(s.solve) for s in slots)
@@ -153,7 +153,7 @@ class Server:
             groups = {}
             for slot in active:
                 groups.setdefault((slot.req.n, slot.precision, slot.req.kind), []).append(slot)
-            launched = []
+            launched, tick0 = [], time.perf_counter()
             for members in groups.values():
                 batch = self._batch_for(members)
                 batch.launch(self.check_every, graph=self.graph)
@@ -161,7 +161,8 @@ class Server:
             for batch in launched:
                 batch.wait()
             if not realtime:
-                virtual += 0.0
+                # without sleeping for arrivals, time still passes while the device works
+                virtual += time.perf_counter() - tick0
             # progress, convergence and completion
             finished = []
             for slot in active:
@@ -169,13 +170,18 @@ class Server:
                 self.grid_point_steps += slot.req.n ** 3 * self.check_every
                 req = slot.req
                 if req.kind == "imaginary":
-                    mu = mu_from_norms(slot.solve.norm_history(self.check_every), req.dt)
+                    try:
+                        mu = mu_from_norms(slot.solve.norm_history(self.check_every), req.dt)
+                    except FloatingPointError as err:
+                        raise FloatingPointError(f"request {req.id} (n={req.n}, {slot.precision}, "
+                                                 f"{slot.steps} steps): {err}") from None
                     slot.dmu = float(abs(mu[-1] - mu[-2]))
-                    if slot.precision == "c64" and slot.dmu < self.switch_tol:
-                        self._promote(slot)
-                    elif slot.precision == "c128" and slot.dmu < req.tol:
+                    if slot.steps >= req.max_steps:
                         finished.append(slot)
-                    elif slot.steps >= req.max_steps:
+                    elif slot.precision == "c64":
+                        if slot.switch.update(slot.steps, slot.dmu):
+                            self._promote(slot)
+                    elif slot.dmu < req.tol:
                         finished.append(slot)
                 elif slot.steps >= req.steps:
                     finished.append(slot)
@@ -209,7 +215,7 @@ class Server:
         diag = slot.solve.diagnostics(req.steps * req.dt if req.kind == "real" else 0.0, spectrum=False)
         self._drop_batches(slot.solve)
         slot.solve.close()
-        converged = req.kind == "real" or slot.dmu < req.tol
+        converged = req.kind == "real" or (slot.precision == "c128" and slot.dmu < req.tol)
         return Outcome(id=req.id, n=req.n, kind=req.kind, precision=req.precision, latency=now - req.arrival,
                        queued=slot.admitted - req.arrival, steps=slot.steps, mu=diag["mu"], energy=diag["E"],
                        norm=diag["norm"], dmu=slot.dmu, converged=converged)

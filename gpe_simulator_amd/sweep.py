@@ -3,9 +3,10 @@
 Many independent solves of one grid side advance together: every pass is one
 launch over all members (gpe_batch_*), and the check_every-step launch
 sequence is replayed from a captured hipGraph.  The bulk of the relaxation
-runs with complex64 fields and FP64 reductions; a member whose chemical
-potential has settled below the complex64 noise floor (switch_tol) is copied
-into a complex128 field and polished, still batched, until |dmu| < tol.
+runs with complex64 fields and FP64 reductions; when its Switch says so, a
+member is copied into a complex128 field and polished, still batched, until
+|dmu| < tol.  The complex64 and complex128 groups are launched on their own
+streams each round and overlap on the device.
 
 Convergence uses the per-step norm the engine records before renormalising:
 starting from a normalised state, one imaginary-time step of length dtau
@@ -55,7 +56,55 @@ class Result:
 
 
 def mu_from_norms(norms, dt):
-    return -np.log(np.asarray(norms)) / (2.0 * dt)
+    norms = np.asarray(norms)
+    if not np.all(np.isfinite(norms) & (norms > 0)):
+        raise FloatingPointError(f"recorded norms must be finite and positive, got {norms[~(norms > 0)][:4]} "
+                                 f"at positions {np.flatnonzero(~(norms > 0))[:4]} of {norms.size}")
+    return -np.log(norms) / (2.0 * dt)
+
+
+class Switch:
+    """When to move a complex64 imaginary-time solve to complex128.
+
+    The norm-decay estimate is exact up to rounding, so in complex64 |dmu|
+    falls exponentially (the slowest excited mode decays as exp(-2 gap t))
+    until it meets a rounding floor (about 1e-6 in reference units at
+    dtau = 5e-4), while the field itself keeps relaxing below that floor.
+    Once the floor shows (no new minimum in `plateau` checks), the clean
+    part of the history (|dmu| > 30x the floor) is fitted by a line in
+    log|dmu| and the solve stays in complex64 until the step where the
+    line reaches `tol`; the complex128 polish then only has to confirm
+    |dmu| < tol with its own, noise-free estimate.  Without a usable fit the
+    switch happens at once."""
+
+    def __init__(self, tol, plateau=10, clean_factor=30.0, fit_points=6):
+        self.tol, self.plateau, self.clean_factor, self.fit_points = tol, plateau, clean_factor, fit_points
+        self.steps, self.dmu = [], []
+        self.target = None
+
+    def update(self, steps, dmu):
+        """Record the check at `steps`; True when the solve should switch."""
+        self.steps.append(steps)
+        self.dmu.append(dmu)
+        if dmu < self.tol:
+            return True
+        if self.target is None:
+            p = self.plateau
+            if len(self.dmu) <= p or min(self.dmu[-p:]) < min(self.dmu[:-p]):
+                return False
+            self.target = self.predict()
+        return steps >= self.target
+
+    def predict(self):
+        floor = float(np.median(self.dmu[-self.plateau:]))
+        clean = [(s, d) for s, d in zip(self.steps, self.dmu) if d > self.clean_factor * floor][-self.fit_points:]
+        if len(clean) < 3:
+            return self.steps[-1]
+        s, d = np.array(clean, dtype=float).T
+        slope, icpt = np.polyfit(s, np.log(d), 1)
+        if not slope < 0:
+            return self.steps[-1]
+        return int(np.ceil((np.log(self.tol) - icpt) / slope))
 
 
 def _new_solve(req, precision, ctx=None):
@@ -88,63 +137,72 @@ def solve_sequential(req, precision="c128", check_every=50, ctx=None):
     return res
 
 
-def sweep_batched(reqs, check_every=50, switch_tol=1e-6, precision="c64", polish="c128", graph=True, ctx=None,
-                  plateau_checks=20):
-    """Relax every request to |dmu| < its tol; returns one Result per request."""
+def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=True, ctx=None, plateau_checks=10):
+    """Relax every request to |dmu| < its tol; returns one Result per request.
+
+    With `polish` set, each member runs in `precision` until its Switch says
+    so, then in `polish` until |dmu| < tol; without it, in `precision` only.
+    Every round advances each group by check_every steps."""
     t0 = time.perf_counter()
     ctx = ctx or _lib.default_context()
     if len({(r.n, r.length) for r in reqs}) != 1:
         raise ValueError("a sweep batches one grid; group requests by side and box first")
+    phases = [precision] + ([polish] if polish else [])
     results = [Result() for _ in reqs]
+    switches = [Switch(r.tol, plateau_checks) for r in reqs]
+    phase_of = [0] * len(reqs)
     solves = []
     for req in reqs:
         s = _new_solve(req, precision, ctx)
         s.gaussian(req.sigma)
         solves.append(s)
-    phases = [(precision, switch_tol if polish else None)]
-    if polish:
-        phases.append((polish, None))
     active = list(range(len(reqs)))
-    for phase, (prec, phase_tol) in enumerate(phases):
-        batch, members = None, None
-        while active:
-            if members != active:
-                if batch is not None:
-                    batch.close()
-                members = list(active)
-                batch = _lib.Batch([solves[i] for i in members], ctx=ctx)
-            batch.step(check_every, graph=graph)
-            still = []
+    batches = {}                       # phase -> (member tuple, Batch)
+    while active:
+        launched = []
+        for ph in range(len(phases)):
+            members = tuple(i for i in active if phase_of[i] == ph)
+            if not members:
+                continue
+            if ph not in batches or batches[ph][0] != members:
+                if ph in batches:
+                    batches.pop(ph)[1].close()
+                batches[ph] = (members, _lib.Batch([solves[i] for i in members], ctx=ctx))
+            batches[ph][1].launch(check_every, graph=graph)
+            launched.append((ph, members))
+        for ph, _ in launched:
+            batches[ph][1].wait()
+        still, promote = [], []
+        for ph, members in launched:
+            bulk = ph + 1 < len(phases)
             for i in members:
                 req, res = reqs[i], results[i]
                 res.steps += check_every
-                if phase == 0:
+                if ph == 0 and bulk:
                     res.steps_c64 += check_every
                 mu = mu_from_norms(solves[i].norm_history(check_every), req.dt)
                 res.mu_estimate, res.dmu = float(mu[-1]), float(abs(mu[-1] - mu[-2]))
-                limit = phase_tol if phase_tol is not None else req.tol
-                if res.dmu < limit or res.steps >= req.max_steps:
+                res.history.append(res.dmu)
+                if res.steps >= req.max_steps:
                     continue
-                if phase_tol is not None:
-                    # leave the complex64 phase once dmu stops falling (its noise floor)
-                    res.history.append(res.dmu)
-                    recent = res.history[-plateau_checks:]
-                    if len(res.history) > plateau_checks and min(recent) >= min(res.history[:-plateau_checks]):
-                        continue
+                if bulk and switches[i].update(res.steps, res.dmu):
+                    promote.append(i)
+                elif not bulk and res.dmu < req.tol:
+                    continue
                 still.append(i)
-            active = still
-        if batch is not None:
-            batch.close()
-        if phase + 1 < len(phases):
-            nxt = phases[phase + 1][0]
-            for i in range(len(reqs)):
-                s = _new_solve(reqs[i], nxt, ctx)
-                s.copy_from(solves[i])
-                solves[i].close()
-                solves[i] = s
-            active = [i for i in range(len(reqs)) if results[i].steps < reqs[i].max_steps]
+        for i in promote:
+            for ph in [p for p, (m, _) in batches.items() if i in m]:
+                batches.pop(ph)[1].close()
+            phase_of[i] += 1
+            s = _new_solve(reqs[i], phases[phase_of[i]], ctx)
+            s.copy_from(solves[i])
+            solves[i].close()
+            solves[i] = s
+        active = [i for i in active if i in set(still)]
+    for _, b in batches.values():
+        b.close()
     for i, req in enumerate(reqs):
-        results[i].converged = results[i].dmu < req.tol
+        results[i].converged = results[i].dmu < req.tol and phase_of[i] == len(phases) - 1
         _finish(solves[i], results[i], t0)
         solves[i].close()
     return results

@@ -47,6 +47,37 @@ def test_batch_members_match_solo_runs(kind, precision, graph, require_gpu):
     batch.close()
 
 
+def test_fresh_member_history_while_another_batch_runs(require_gpu):
+    """Buffers a solve allocates on its first batched step (norm history,
+    partials) are cleared on the solve's own stream, so a long batch on
+    another stream cannot delay the clear past the new member's writes."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve
+    x, k = orc.get_grids(128, 20.0)
+    busy = [make_solve(128, x, k, "real", 1e-4, 100.0, static="x**2 + y**2 + z**2", precision="c64")
+            for _ in range(4)]
+    for s in busy:
+        s.gaussian(1.0)
+    long_batch = _lib.Batch(busy)
+    x, k = orc.get_grids(64, 16.0)
+    for trial in range(3):
+        long_batch.launch(400, graph=True)
+        fresh = members("imaginary", "c64", count=2)
+        alone = members("imaginary", "c64", count=2)
+        for a, b in zip(fresh, alone):
+            a.gaussian(1.0)
+            b.gaussian(1.0)
+        batch = _lib.Batch(fresh)
+        batch.launch(50, graph=True)
+        batch.wait()
+        long_batch.wait()
+        for a, b in zip(fresh, alone):
+            b.step(50)
+            np.testing.assert_array_equal(a.norm_history(50), b.norm_history(50))
+        batch.close()
+    long_batch.close()
+
+
 def test_batch_rejects_mixed_shapes(require_gpu):
     from gpe_simulator_amd import _lib
     a = members("imaginary", "c64", n=64, count=1)[0]
