@@ -110,22 +110,26 @@ def serving(count, seed=0):
     return stats
 
 
-def sweep(count, sequential_sample=4, seed=0):
+def sweep(count, every=4, seed=0):
     """64 ground states at 96^3 (seeded g, omega_z, widths) batched, against
-    sequential complex128 solves of the first `sequential_sample` requests
-    (the sequential time is extrapolated to all requests)."""
+    sequential complex128 solves of every `every`-th request (a sample spread
+    over the draw; the sequential time is extrapolated to all requests)."""
     from gpe_simulator_amd.sweep import config_sweep, solve_sequential, sweep_batched
     reqs = config_sweep(count=count, seed=seed)
     t0 = time.perf_counter()
     res = sweep_batched(reqs)
     batched = time.perf_counter() - t0
+    sample = list(range(0, count, every))
     t0 = time.perf_counter()
-    seq = [solve_sequential(r) for r in reqs[:sequential_sample]]
-    per_solve = (time.perf_counter() - t0) / sequential_sample
+    seq = {i: solve_sequential(reqs[i]) for i in sample}
+    per_solve = (time.perf_counter() - t0) / len(sample)
     return {"requests": count, "grid": reqs[0].n, "batched_s": batched,
+            "sequential_sample": f"requests {sample[0]}, {sample[1]}, ... every {every}th ({len(sample)} solves)",
             "sequential_s_est": per_solve * count, "speedup": per_solve * count / batched,
             "converged": sum(r.converged for r in res),
-            "mu_rel_diff_vs_sequential": max(abs(a.mu - b.mu) / abs(b.mu) for a, b in zip(res, seq))}
+            "steps_batched_mean": float(np.mean([r.steps for r in res])),
+            "steps_sequential_mean": float(np.mean([s.steps for s in seq.values()])),
+            "mu_rel_diff_vs_sequential": max(abs(res[i].mu - s.mu) / abs(s.mu) for i, s in seq.items())}
 
 
 def cpu_baseline(psi0, samples):

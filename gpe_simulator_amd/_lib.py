@@ -31,6 +31,8 @@ EXPORTS = (
     "gpe_norm_history", "gpe_norm", "gpe_normalise", "gpe_diagnostics", "gpe_sync",
     "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
+    "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
+    "gpe_solve_get_variants",
 )
 
 
@@ -92,6 +94,11 @@ def _declare(lib):
         "gpe_batch_step_timed": (c_int, [vp, i64, dp, POINTER(i64)]),
         "gpe_batch_launch": (c_int, [vp, i64, i32]),
         "gpe_batch_wait": (c_int, [vp]),
+        "gpe_pass_variants": (c_int, [i32, i32, POINTER(i32)]),
+        "gpe_pass_variant_info": (c_int, [i32, i32, i32, POINTER(i32), POINTER(i32), POINTER(i32)]),
+        "gpe_ctx_set_variants": (c_int, [vp, i32, i32, i32, POINTER(i32), i32]),
+        "gpe_solve_set_variants": (c_int, [vp, POINTER(i32), i32]),
+        "gpe_solve_get_variants": (c_int, [vp, POINTER(i32), i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -141,11 +148,19 @@ class Context:
         check(lib().gpe_ctx_create(int(device), byref(handle)))
         self.handle = handle
         self.device = device
+        self.variant_defaults = {}     # (side, precision, kind) -> variants set on this context
+        self.tunings = {}              # autotuner results cached on this context
 
     def mem_info(self):
         free, total = c_size_t(), c_size_t()
         check(lib().gpe_mem_info(self.handle, byref(free), byref(total)))
         return free.value, total.value
+
+    def set_variants(self, n, precision, kind, variants):
+        """Pass variants (one per kind in PASS_KINDS) for solves created later."""
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        check(lib().gpe_ctx_set_variants(self.handle, int(n), int(precision), int(kind), i32ptr(v), v.size))
+        self.variant_defaults[(int(n), int(precision), int(kind))] = [int(x) for x in v]
 
     def bench_copy(self, nbytes, reps=20):
         out = c_double()
@@ -186,6 +201,19 @@ def footprint(n, precision, kind=REAL):
     resident, diag = c_size_t(), c_size_t()
     check(lib().gpe_solve_footprint(byref(desc), byref(resident), byref(diag)))
     return resident.value, diag.value
+
+
+def pass_variants(n, precision):
+    """[(threads per pencil, pencils per block, waves per SIMD)] of the
+    compiled pass variants of (side, precision)."""
+    count = c_int32()
+    check(lib().gpe_pass_variants(int(n), int(precision), byref(count)))
+    out = []
+    for v in range(count.value):
+        tp, b, w = c_int32(), c_int32(), c_int32()
+        check(lib().gpe_pass_variant_info(int(n), int(precision), v, byref(tp), byref(b), byref(w)))
+        out.append((tp.value, b.value, w.value))
+    return out
 
 
 def nbins_for(n):
@@ -284,6 +312,19 @@ class Solve:
         check(lib().gpe_step_timed(self.handle, int(nsteps), dptr(tm) if tm is not None else None, dptr(ms),
                                    i64ptr(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def set_variants(self, variants):
+        """Pass variant per kind (sequence in PASS_KINDS order, or a dict)."""
+        if isinstance(variants, dict):
+            current = self.variants()
+            variants = [variants.get(k, current[i]) for i, k in enumerate(PASS_KINDS)]
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        check(lib().gpe_solve_set_variants(self.handle, i32ptr(v), v.size))
+
+    def variants(self):
+        v = np.zeros(len(PASS_KINDS), dtype=np.int32)
+        check(lib().gpe_solve_get_variants(self.handle, i32ptr(v), v.size))
+        return [int(x) for x in v]
 
     def norm_history(self, count):
         out = np.empty(int(count))

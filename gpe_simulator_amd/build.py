@@ -2,8 +2,9 @@
 
 Usage: python -m gpe_simulator_amd.build [--jobs N]
 
-Every pass instantiation (precision x grid side) is its own translation unit
-so the build runs in parallel.  Objects go to gpe_simulator_amd/_build/ and
+Every pass instantiation (precision x grid side x tile variant, read from the
+variant list in csrc/gpe_shapes.h) is its own translation unit so the build
+runs in parallel.  Objects go to gpe_simulator_amd/_build/ and
 the shared library to gpe_simulator_amd/libgpe_hip.so.
 """
 
@@ -12,6 +13,7 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as futures
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -23,7 +25,6 @@ CSRC = PACKAGE / "csrc"
 BUILD = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libgpe_hip.so"
 ARCH = "gfx950"
-FAST_SIDES = (0, 64, 96, 128, 192, 256)
 
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -40,13 +41,22 @@ def hipcc() -> str:
     return path
 
 
+def variants():
+    """(precision, side, variant) of every compiled pass variant: the generic
+    transform of each precision plus the X(...) lines of gpe_shapes.h."""
+    text = (CSRC / "gpe_shapes.h").read_text()
+    block = text[text.index("#define GPE_VARIANTS(X)"):text.index("namespace gpe")]
+    found = [tuple(int(v) for v in m) for m in
+             re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*\d+,\s*\d+,\s*\d+\)", block)]
+    return [(0, 0, 0), (1, 0, 0)] + [f[:3] for f in found]
+
+
 def _units():
     units = [(CSRC / "gpe_api.hip", BUILD / "gpe_api.o", []),
              (CSRC / "gpe_misc.hip", BUILD / "gpe_misc.o", [])]
-    for prec in (0, 1):
-        for side in FAST_SIDES:
-            units.append((CSRC / "gpe_pass_inst.hip", BUILD / f"gpe_pass_p{prec}_n{side}.o",
-                          [f"-DGPE_PREC={prec}", f"-DGPE_N={side}"]))
+    for prec, side, var in variants():
+        units.append((CSRC / "gpe_pass_inst.hip", BUILD / f"gpe_pass_p{prec}_n{side}_v{var}.o",
+                      [f"-DGPE_PREC={prec}", f"-DGPE_N={side}", f"-DGPE_V={var}"]))
     return units
 
 

@@ -3,9 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gpe_hip.h"
@@ -56,31 +59,47 @@ static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<double>), "pass argument la
 
 int nbins_for(int n) { return int(std::floor(std::sqrt(3.0) * (n / 2) + 0.5)) + 1; }
 
-bool is_fast_side(int n) { return n == 64 || n == 96 || n == 128 || n == 192 || n == 256; }
+// Compiled pass variants.  Entries of one (precision, side) are contiguous and
+// ordered by variant index; sides without a fast transform use the generic
+// entry of their precision.
+struct PassVariant {
+  int prec, n, v, tp, b, waves;
+  PassLauncher launch;
+  PassBlocks blocks;
+};
 
-void pick_launchers(int prec, int n, PassLauncher* l, PassBlocks* b) {
-#define GPE_PICK(P, N)            \
-  if (prec == P && n == N) {      \
-    *l = launch_pass_p##P##_n##N; \
-    *b = pass_blocks_p##P##_n##N; \
-    return;                       \
+const PassVariant kVariantTable[] = {
+    {0, 0, 0, 64, 1, 1, launch_pass_p0_n0_v0, pass_blocks_p0_n0_v0},
+    {1, 0, 0, 64, 1, 1, launch_pass_p1_n0_v0, pass_blocks_p1_n0_v0},
+#define GPE_VARIANT_ENTRY(P, N, V, TP, B, W) \
+  {P, N, V, TP, B, W, launch_pass_p##P##_n##N##_v##V, pass_blocks_p##P##_n##N##_v##V},
+    GPE_VARIANTS(GPE_VARIANT_ENTRY)
+#undef GPE_VARIANT_ENTRY
+};
+
+// First variant and count for (precision, side).
+int variants_for(int prec, int n, const PassVariant** first) {
+  const int total = int(sizeof(kVariantTable) / sizeof(kVariantTable[0]));
+  int count = 0;
+  for (int i = 0; i < total; ++i) {
+    if (kVariantTable[i].prec == prec && kVariantTable[i].n == n) {
+      if (count == 0) *first = &kVariantTable[i];
+      ++count;
+    }
   }
-  GPE_PICK(0, 64) GPE_PICK(0, 96) GPE_PICK(0, 128) GPE_PICK(0, 192) GPE_PICK(0, 256)
-  GPE_PICK(1, 64) GPE_PICK(1, 96) GPE_PICK(1, 128) GPE_PICK(1, 192) GPE_PICK(1, 256)
-#undef GPE_PICK
-  if (prec == 0) {
-    *l = launch_pass_p0_n0;
-    *b = pass_blocks_p0_n0;
-  } else {
-    *l = launch_pass_p1_n0;
-    *b = pass_blocks_p1_n0;
+  if (count == 0) {
+    *first = &kVariantTable[prec == 0 ? 0 : 1];
+    count = 1;
   }
+  return count;
 }
 
 }  // namespace
 
 struct gpe_ctx {
   int device = 0;
+  // per-pass-kind variant defaults for new solves, keyed by (side, precision, kind)
+  std::map<std::tuple<int, int, int>, std::array<int, kPassKinds>> variants;
 };
 
 struct gpe_solve {
@@ -90,9 +109,13 @@ struct gpe_solve {
   long long n3 = 0;
   size_t esz = 0, rsz = 0;
   hipStream_t st = nullptr;
-  PassLauncher launch = nullptr;
-  PassBlocks blocks_fn = nullptr;
-  long long blocks(int kind, int n) const { return blocks_fn(kind, n); }
+  const PassVariant* vt = nullptr;        // compiled variants of this (precision, side)
+  int nvt = 0;
+  int var[kPassKinds] = {0};              // variant used by each pass kind
+  long long blocks(int kind, int n) const { return vt[var[kind]].blocks(kind, n); }
+  hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream) const {
+    return vt[var[kind]].launch(kind, args, n, batch, stream);
+  }
 
   void* psi = nullptr;
   void* work = nullptr;
@@ -125,6 +148,7 @@ struct gpe_solve {
   long long hist_cap = 0, hist_len = 0;
   double* nkocc = nullptr;
   unsigned int* nkcnt = nullptr;
+  long long nk_cap = 0;                   // blocks the nk partial buffers hold
   double* d_out = nullptr;
   double* d_nkocc = nullptr;
   long long* d_nkcnt = nullptr;
@@ -522,22 +546,34 @@ int norm_t(gpe_solve* s) {
 int norm_any(gpe_solve* s) { return s->prec == GPE_C64 ? norm_t<float>(s) : norm_t<double>(s); }
 
 int ensure_diag(gpe_solve* s) {
-  if (s->work) return GPE_OK;
-  CHK(dzalloc(&s->work, s->n3 * s->esz, s->st));
+  bool changed = false;
+  if (!s->work) {
+    CHK(dzalloc(&s->work, s->n3 * s->esz, s->st));
+    CHK(dzalloc(&s->d_out, 16 * sizeof(double), s->st));
+    CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double), s->st));
+    CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long), s->st));
+    changed = true;
+  }
+  // partial buffers follow the blocks of the pass variants in use
   const long long bx = s->blocks(K_X_DIAG, s->n);
-  CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double), s->st));
-  CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int), s->st));
-  CHK(dzalloc(&s->d_out, 16 * sizeof(double), s->st));
-  CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double), s->st));
-  CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long), s->st));
+  if (bx > s->nk_cap) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    dfree(s->nkocc);
+    dfree(s->nkcnt);
+    s->nkocc = nullptr;
+    s->nkcnt = nullptr;
+    s->nk_cap = 0;
+    CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double), s->st));
+    CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int), s->st));
+    s->nk_cap = bx;
+    changed = true;
+  }
   const long long need = std::max(s->blocks(K_Z_DIAG, s->n) * 3, bx * 2);
   if (need > s->red_cap) {
-    dfree(s->red);
-    s->red = nullptr;
-    CHK(dzalloc(&s->red, need * sizeof(double), s->st));
-    s->red_cap = need;
+    CHK(ensure_red(s, need));
+    changed = false;   // ensure_red refreshed the descriptors
   }
-  return refresh_desc(s);
+  return changed ? refresh_desc(s) : GPE_OK;
 }
 
 int ensure_stage(gpe_solve* s) {
@@ -583,6 +619,65 @@ int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
   return GPE_OK;
 }
 
+int gpe_pass_variants(int32_t n, int32_t precision, int32_t* count) {
+  if (!count) return fail(GPE_ERR_INVALID, "null argument");
+  if (precision != GPE_C64 && precision != GPE_C128) return fail(GPE_ERR_INVALID, "bad precision");
+  const PassVariant* first = nullptr;
+  *count = variants_for(precision, n, &first);
+  return GPE_OK;
+}
+
+int gpe_pass_variant_info(int32_t n, int32_t precision, int32_t variant, int32_t* threads_per_pencil,
+                          int32_t* pencils_per_block, int32_t* waves) {
+  if (!threads_per_pencil || !pencils_per_block || !waves) return fail(GPE_ERR_INVALID, "null argument");
+  if (precision != GPE_C64 && precision != GPE_C128) return fail(GPE_ERR_INVALID, "bad precision");
+  const PassVariant* first = nullptr;
+  const int count = variants_for(precision, n, &first);
+  if (variant < 0 || variant >= count) return fail(GPE_ERR_INVALID, "variant out of range");
+  *threads_per_pencil = first[variant].tp;
+  *pencils_per_block = first[variant].b;
+  *waves = first[variant].waves;
+  return GPE_OK;
+}
+
+int gpe_solve_set_variants(gpe_solve* s, const int32_t* variants, int32_t nkinds) {
+  if (!s || !variants) return fail(GPE_ERR_INVALID, "null argument");
+  if (nkinds != kPassKinds) return fail(GPE_ERR_INVALID, "nkinds must be " + std::to_string(kPassKinds));
+  for (int k = 0; k < kPassKinds; ++k)
+    if (variants[k] < 0 || variants[k] >= s->nvt) return fail(GPE_ERR_INVALID, "variant out of range");
+  CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  std::copy(variants, variants + kPassKinds, s->var);
+  // partial buffers are sized from the block counts of the variants in use
+  long long need = kNormBlocks;
+  for (int k = 0; k < kPassKinds; ++k) need = std::max(need, s->blocks(k, s->n) * 3);
+  return ensure_red(s, need);
+}
+
+int gpe_solve_get_variants(gpe_solve* s, int32_t* variants, int32_t nkinds) {
+  if (!s || !variants) return fail(GPE_ERR_INVALID, "null argument");
+  if (nkinds != kPassKinds) return fail(GPE_ERR_INVALID, "nkinds must be " + std::to_string(kPassKinds));
+  std::copy(s->var, s->var + kPassKinds, variants);
+  return GPE_OK;
+}
+
+int gpe_ctx_set_variants(gpe_ctx* ctx, int32_t n, int32_t precision, int32_t kind, const int32_t* variants,
+                         int32_t nkinds) {
+  if (!ctx || !variants) return fail(GPE_ERR_INVALID, "null argument");
+  if (nkinds != kPassKinds) return fail(GPE_ERR_INVALID, "nkinds must be " + std::to_string(kPassKinds));
+  if (precision != GPE_C64 && precision != GPE_C128) return fail(GPE_ERR_INVALID, "bad precision");
+  if (kind != GPE_REAL && kind != GPE_IMAG) return fail(GPE_ERR_INVALID, "bad kind");
+  const PassVariant* first = nullptr;
+  const int count = variants_for(precision, n, &first);
+  std::array<int, kPassKinds> v{};
+  for (int k = 0; k < kPassKinds; ++k) {
+    if (variants[k] < 0 || variants[k] >= count) return fail(GPE_ERR_INVALID, "variant out of range");
+    v[k] = variants[k];
+  }
+  ctx->variants[std::make_tuple(n, precision, kind)] = v;
+  return GPE_OK;
+}
+
 int gpe_solve_footprint(const gpe_solve_desc* d, size_t* resident, size_t* diag) {
   if (!d || !resident || !diag) return fail(GPE_ERR_INVALID, "null argument");
   const size_t n = (size_t)d->n, n3 = n * n * n;
@@ -621,7 +716,12 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   s->nbins = nbins_for(d->n);
   const double dx = d->x_array[1] - d->x_array[0];
   s->dx3 = dx * dx * dx;
-  pick_launchers(s->prec, s->n, &s->launch, &s->blocks_fn);
+  s->nvt = variants_for(s->prec, s->n, &s->vt);
+  {
+    auto it = ctx->variants.find(std::make_tuple(s->n, s->prec, s->kind));
+    if (it != ctx->variants.end())
+      for (int k = 0; k < kPassKinds; ++k) s->var[k] = it->second[k] < s->nvt ? it->second[k] : 0;
+  }
 
   auto bail = [&](int code) {
     gpe_solve_destroy(s);
@@ -960,10 +1060,10 @@ struct gpe_batch {
   int n = 0, prec = 0, kind = 0;
   hipStream_t st = nullptr;
   void* desc = nullptr;                 // device array of per-solve descriptors
-  PassLauncher launch = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
+  int graph_var[kPassKinds] = {0};      // pass variants the graph was captured with
   bool timing = false;
   hipEvent_t ev[2] = {nullptr, nullptr};
   double ms[kPassKinds] = {0};
@@ -1002,7 +1102,6 @@ int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_
   b->n = solves[0]->n;
   b->prec = solves[0]->prec;
   b->kind = solves[0]->kind;
-  b->launch = solves[0]->launch;
   auto bail = [&](int code) {
     gpe_batch_destroy(b);
     return code;
@@ -1023,6 +1122,9 @@ namespace {
 int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
   const size_t dsz = b->prec == GPE_C64 ? sizeof(SolveDev<float>) : sizeof(SolveDev<double>);
   *prescale_first = false;
+  for (gpe_solve* s : b->solves)
+    if (!std::equal(s->var, s->var + kPassKinds, b->solves[0]->var))
+      return fail(GPE_ERR_INVALID, "batch members must use the same pass variants");
   for (gpe_solve* s : b->solves) {
     if (b->kind == GPE_IMAG) {
       CHK(ensure_hist(s, nsteps));
@@ -1049,7 +1151,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   PassArgsRaw a{b->desc, sa, sb, flags, hist};
   if (b->timing) HIPCHK(hipEventRecord(b->ev[0], b->st));
-  hipError_t e = b->launch(kind, &a, b->n, (int)b->solves.size(), b->st);
+  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
   if (b->timing) {
     HIPCHK(hipEventRecord(b->ev[1], b->st));
@@ -1090,7 +1192,8 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   if (!use_graph || b->timing) {
     CHK(emit());
   } else {
-    if (!b->exec || b->graph_steps != nsteps) {
+    const int* var = b->solves[0]->var;
+    if (!b->exec || b->graph_steps != nsteps || !std::equal(var, var + kPassKinds, b->graph_var)) {
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
@@ -1107,6 +1210,7 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
       b->graph = g;
       HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->graph_steps = nsteps;
+      std::copy(var, var + kPassKinds, b->graph_var);
     }
     HIPCHK(hipGraphLaunch(b->exec, b->st));
   }

@@ -2,6 +2,8 @@
 // launchers compiled from gpe_pass_inst.hip.
 #pragma once
 
+#include "gpe_shapes.h"
+
 #include "gpe_passes.h"
 #include "gpe_vm.h"
 
@@ -34,24 +36,17 @@ void launch_finish_nk(const double* occ, const unsigned int* cnt, long long nblk
                       long long* out_cnt, hipStream_t st);
 void launch_copy(const void* in, void* out, long long bytes, hipStream_t st);
 
-// Pass launchers: one per (precision, side) instantiation; side 0 = generic.
+// Pass launchers: one per (precision, side, variant) instantiation; side 0 is
+// the generic transform (one variant).
 using PassLauncher = hipError_t (*)(int kind, const void* args, int n, int batch, hipStream_t st);
 using PassBlocks = long long (*)(int kind, int n);
 
-#define GPE_DECL_PASS(P, N)                                                              \
-  hipError_t launch_pass_p##P##_n##N(int kind, const void* args, int n, int batch, hipStream_t st); \
-  long long pass_blocks_p##P##_n##N(int kind, int n);
-GPE_DECL_PASS(0, 0)
-GPE_DECL_PASS(0, 64)
-GPE_DECL_PASS(0, 96)
-GPE_DECL_PASS(0, 128)
-GPE_DECL_PASS(0, 192)
-GPE_DECL_PASS(0, 256)
-GPE_DECL_PASS(1, 0)
-GPE_DECL_PASS(1, 64)
-GPE_DECL_PASS(1, 96)
-GPE_DECL_PASS(1, 128)
-GPE_DECL_PASS(1, 192)
-GPE_DECL_PASS(1, 256)
+#define GPE_DECL_PASS(P, N, V, ...)                                                                 \
+  hipError_t launch_pass_p##P##_n##N##_v##V(int kind, const void* args, int n, int batch, hipStream_t st); \
+  long long pass_blocks_p##P##_n##N##_v##V(int kind, int n);
+GPE_DECL_PASS(0, 0, 0)
+GPE_DECL_PASS(1, 0, 0)
+GPE_VARIANTS(GPE_DECL_PASS)
+#undef GPE_DECL_PASS
 
 }  // namespace gpe

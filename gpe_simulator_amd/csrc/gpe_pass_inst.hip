@@ -1,7 +1,8 @@
-// One translation unit per (precision, grid size): the build compiles this
-// file once for each GPE_PREC (0 = complex64, 1 = complex128) and GPE_N
-// (0 = generic direct-DFT transform, otherwise a fast pencil shape), so the
-// heavy template instantiations compile in parallel.
+// One translation unit per (precision, grid size, variant): the build compiles
+// this file once for each GPE_PREC (0 = complex64, 1 = complex128), GPE_N
+// (0 = generic direct-DFT transform, otherwise a fast side) and GPE_V (tile
+// variant from the list in gpe_shapes.h), so the heavy template
+// instantiations compile in parallel.
 #include "gpe_passes.h"
 #include "gpe_shapes.h"
 
@@ -10,6 +11,9 @@
 #endif
 #ifndef GPE_N
 #error "GPE_N must be defined"
+#endif
+#ifndef GPE_V
+#error "GPE_V must be defined"
 #endif
 
 namespace gpe {
@@ -23,12 +27,13 @@ using Real = double;
 #if GPE_N == 0
 using TF = GenTF<Real>;
 #else
-using TF = typename Shape<Real, GPE_N>::TF;
+using TF = typename Shape<GPE_PREC, GPE_N, GPE_V>::TF;
 #endif
 
 #define GPE_CAT2(a, b) a##b
 #define GPE_CAT(a, b) GPE_CAT2(a, b)
-#define GPE_LAUNCHER GPE_CAT(GPE_CAT(GPE_CAT(launch_pass_p, GPE_PREC), _n), GPE_N)
+#define GPE_SUFFIX GPE_CAT(GPE_CAT(GPE_CAT(GPE_CAT(GPE_CAT(p, GPE_PREC), _n), GPE_N), _v), GPE_V)
+#define GPE_LAUNCHER GPE_CAT(launch_pass_, GPE_SUFFIX)
 
 template <int AXIS, int KIND>
 static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a) {
@@ -71,7 +76,7 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
 }
 
 // Blocks per solve that a pass of `kind` launches (sizes partial buffers).
-long long GPE_CAT(GPE_CAT(GPE_CAT(pass_blocks_p, GPE_PREC), _n), GPE_N)(int kind, int n) {
+long long GPE_CAT(pass_blocks_, GPE_SUFFIX)(int kind, int n) {
   const int axis = axis_of(kind);
   return axis == 2 ? tile_count<TF, 2>(n) : (axis == 1 ? tile_count<TF, 1>(n) : tile_count<TF, 0>(n));
 }

@@ -11,7 +11,8 @@
 //   X pass  (stride n^2):       FFT_x -> kinetic factor -> inverse FFT_x.
 //
 // Two pencil transforms share the same pass code:
-//   FastTF<R, N, TP, B>: N = TP * M with TP threads per pencil; each thread
+//   FastTF<R, N, TP, B, W>: N = TP * M with TP threads per pencil (B pencils
+//     per block, register budget for W waves per SIMD); each thread
 //     holds M values at positions t + TP*m; a length-M codelet in registers,
 //     an FP64-derived twiddle, one LDS exchange and TP-point codelets.  The
 //     output comes back in the same register layout, so FFT -> pointwise ->
@@ -89,16 +90,16 @@ struct PassArgs {
 };
 
 // ------------------------------------------------------------ transforms --
-template <typename R, int N_, int TP_, int B_>
+template <typename R, int N_, int TP_, int B_, int W_>
 struct FastTF {
   static constexpr bool kFast = true;
   static constexpr int N = N_, TP = TP_, B = B_;
   static constexpr int M = N / TP;
   static constexpr int Q = M / TP;
   static constexpr int THREADS = TP * B;
-  // Waves per SIMD the register allocation must allow (LDS caps residency at
-  // about four 256-thread blocks per CU, so more registers buy nothing).
-  static constexpr int MIN_WAVES = sizeof(R) == 4 ? 3 : 2;
+  // Waves per SIMD the register allocation must allow (a tuning parameter:
+  // more waves hide more latency but cap the registers per thread).
+  static constexpr int MIN_WAVES = W_;
   static constexpr int LSTRIDE = TP + 1;           // padded LDS row
   // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
   // pencils, so an even stride would put them on the same LDS bank.
@@ -124,6 +125,9 @@ struct FastTF {
       cpx<double> w = twl[k1 * TP + t];
       if constexpr (DIR > 0) w.y = -w.y;
       v[k1] = cmul_d(v[k1], w);
+      // scheduling fence every 4 twiddles: the FP64 twiddles (4 registers
+      // each) are read from LDS as they are used, not all up front
+      if constexpr (k1 % 4 == 0) __builtin_amdgcn_sched_barrier(0);
     });
     __syncthreads();
     static_for<0, M>([&](auto k1c) {
@@ -303,6 +307,31 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const long long stride = AXIS == 2 ? 1LL : (AXIS == 1 ? (long long)n : (long long)n * n);
   const int ntiles = tile_count<TF, AXIS>(n);
   cpx<R>* lp = lds + b * TF::PS;
+  // Block-shared tables staged next to the twiddles (the twiddle barrier
+  // covers them): the FP64 x kinetic factors for X_KIN and the z potential
+  // line(s) for the Z passes, so they are read from LDS where used instead of
+  // being loaded (and held in registers) ahead of use.
+  constexpr bool kStageKin = TF::kFast && KIND == K_X_KIN;
+  constexpr bool kStagePz = TF::kFast && (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST);
+  __shared__ cpx<double> kin_lds[kStageKin ? TF::N : 1];
+  __shared__ R pz_lds[kStagePz ? 2 * TF::N : 1];
+  if constexpr (kStageKin) {
+    for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) kin_lds[e] = gload(s.kin_x + e);
+  }
+  if constexpr (kStagePz) {
+    for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) {
+      pz_lds[e] = gload(s.pz + (a.slot_a * s.pz_ss + e));
+      if constexpr (KIND == K_Z_MID) pz_lds[TF::N + e] = gload(s.pz + (a.slot_b * s.pz_ss + e));
+    }
+  }
+  auto kin_at = [&](int i) -> cpx<double> {
+    if constexpr (kStageKin) return kin_lds[i];
+    else return gload(s.kin_x + i);
+  };
+  // Elements per scheduling chunk in the kinetic loop: the compiler may not
+  // move work across chunk boundaries, which bounds the FP64 factors it keeps
+  // live at once.
+  constexpr int CHUNK = 4;
   TF::stage_twiddles(twl_lds, s.tw);
   const cpx<double>* twp = TF::kFast ? static_cast<const cpx<double>*>(twl_lds) : s.tw;
   const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
@@ -347,9 +376,10 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           if (TF::valid(t, m, n)) {
-            const double f = gload(s.kin_x + TF::pos(t, m)).x * eyz;
+            const double f = kin_at(TF::pos(t, m)).x * eyz;
             v[m] = cscale_d(v[m], f);
           }
+          if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       } else {
         const cpx<double> eyz = cmul(gload(s.kin_y + pj), gload(s.kin_z + pk));
@@ -362,7 +392,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         for (int m = 0; m < M; ++m) {
           if (TF::valid(t, m, n)) {
             const int i = TF::pos(t, m);
-            cpx<double> f = cmul(gload(s.kin_x + i), eyz);
+            cpx<double> f = cmul(kin_at(i), eyz);
             if (s.masked) {
               const double kx = gload(s.karr + i);
               // same association as kx*kx + ky*ky + kz*kz, without contraction
@@ -371,6 +401,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
             }
             v[m] = cmul_d(v[m], f);
           }
+          if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
       TF::template fft<+1>(v, lp, t, twp, n);
@@ -392,12 +423,16 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           R vst = R(0), vi = R(0);
           if (s.vre) vst = gload(s.vre + (rowoff + k));
           if (has_vim) vi = gload(s.vim + (rowoff + k));
-          const R pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
+          R pza;
+          if constexpr (kStagePz) pza = pz_lds[k];
+          else pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
           R pa = pxa + pza + vst;
           if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
           cpx<R> w;
           if constexpr (KIND == K_Z_MID) {
-            const R pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
+            R pzb;
+            if constexpr (kStagePz) pzb = pz_lds[TF::N + k];
+            else pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
             R pb = pxb + pzb + vst;
             if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
             w = two_half_steps(v[m], pa, pb, vi, has_vim, s);

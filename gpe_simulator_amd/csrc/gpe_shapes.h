@@ -1,30 +1,46 @@
-// Pencil tile shapes per (precision, grid side).  TP threads cooperate on one
-// pencil and hold M = N/TP values each; B pencils share a block.  For the
-// strided X/Y passes the B pencils are adjacent along z, so every load
-// instruction reads B*sizeof(complex) = 128 contiguous bytes per row.
+// Pencil tile variants per (precision, grid side).
+//
+// TP threads cooperate on one pencil and hold M = N/TP values each (N must be
+// TP*TP*Q); B pencils share a block; W is the waves per SIMD the register
+// allocation must allow.  For the strided
+// X/Y passes the B pencils are adjacent along z, so every load instruction
+// reads B*sizeof(complex) contiguous bytes per row; in Z passes a block's tile
+// is B*N contiguous elements.  Each pass kind of a solve can run a different
+// variant (all read and write the same canonical field), and the autotuner
+// picks per kind; variant 0 is the default before tuning.
+//
+// One line per variant: X(precision 0/1 = complex64/complex128, side, variant,
+// TP, B, W).  The build compiles one translation unit per line.
 #pragma once
 
 #include "gpe_passes.h"
 
+#define GPE_VARIANTS(X)                                                                          \
+  X(0, 64, 0, 8, 32, 3) X(0, 64, 1, 8, 16, 4) X(0, 64, 2, 8, 32, 4)                     \
+  X(0, 96, 0, 4, 32, 3) X(0, 96, 1, 4, 16, 3) X(0, 96, 2, 4, 32, 4)                     \
+  X(0, 128, 0, 8, 32, 3) X(0, 128, 1, 8, 16, 3) X(0, 128, 2, 8, 16, 4)                  \
+  X(0, 192, 0, 8, 16, 3) X(0, 192, 1, 8, 32, 3) X(0, 192, 2, 8, 16, 4)                  \
+  X(0, 256, 0, 16, 16, 3) X(0, 256, 1, 16, 16, 4) X(0, 256, 2, 16, 32, 3)               \
+  X(1, 64, 0, 8, 16, 2) X(1, 64, 1, 8, 32, 2) X(1, 64, 2, 4, 16, 2)                     \
+  X(1, 96, 0, 4, 16, 2) X(1, 96, 1, 4, 32, 2) X(1, 96, 2, 4, 8, 2)                      \
+  X(1, 128, 0, 8, 16, 2) X(1, 128, 1, 8, 32, 2) X(1, 128, 2, 8, 8, 2)                   \
+  X(1, 192, 0, 8, 8, 2) X(1, 192, 1, 8, 16, 2)                                             \
+  X(1, 256, 0, 16, 8, 2) X(1, 256, 1, 16, 16, 2) X(1, 256, 2, 16, 4, 2)
+
 namespace gpe {
 
-template <typename R, int N>
+template <int P>
+using prec_t = typename std::conditional<P == 0, float, double>::type;
+
+template <int P, int N, int V>
 struct Shape;
 
-// complex64
-template <> struct Shape<float, 64> { using TF = FastTF<float, 64, 8, 32>; };
-template <> struct Shape<float, 96> { using TF = FastTF<float, 96, 4, 32>; };
-template <> struct Shape<float, 128> { using TF = FastTF<float, 128, 8, 32>; };
-template <> struct Shape<float, 192> { using TF = FastTF<float, 192, 8, 16>; };
-template <> struct Shape<float, 256> { using TF = FastTF<float, 256, 16, 16>; };
-// complex128
-template <> struct Shape<double, 64> { using TF = FastTF<double, 64, 8, 16>; };
-template <> struct Shape<double, 96> { using TF = FastTF<double, 96, 4, 16>; };
-template <> struct Shape<double, 128> { using TF = FastTF<double, 128, 8, 16>; };
-template <> struct Shape<double, 192> { using TF = FastTF<double, 192, 8, 8>; };
-template <> struct Shape<double, 256> { using TF = FastTF<double, 256, 16, 8>; };
-
-// Sides with a fast instantiation; every other side 2..1024 uses GenTF.
-#define GPE_FAST_SIDES(X) X(64) X(96) X(128) X(192) X(256)
+#define GPE_SHAPE_SPEC(P, N, V, TP, B, W) \
+  template <>                             \
+  struct Shape<P, N, V> {                 \
+    using TF = FastTF<prec_t<P>, N, TP, B, W>; \
+  };
+GPE_VARIANTS(GPE_SHAPE_SPEC)
+#undef GPE_SHAPE_SPEC
 
 }  // namespace gpe

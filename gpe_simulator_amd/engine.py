@@ -1,6 +1,7 @@
 """Build device solves from reference-style parameters.
 
-`make_solve` is what `stage.prepare` uses: it creates a libgpe_hip solve,
+`make_solve` is what `stage.prepare` uses: it creates a libgpe_hip solve
+(after tuning the pass variants of its shape once per context, see tune.py),
 plans the static and time-dependent potential expressions into per-target
 programs (see expressions.plan_potential) and registers them.
 """
@@ -9,7 +10,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tune
 from .expressions import TGT_VOL, TGT_XY, TGT_Z, plan_potential
 
 PRECISIONS = {"c64": _lib.C64, "complex64": _lib.C64, "c128": _lib.C128, "complex128": _lib.C128}
@@ -31,6 +32,9 @@ def make_solve(n, x_array, k_array, kind, dt, beta, static=None, dynamic=None, k
     kind: "real" or "imaginary"; static / dynamic: schema-2 expressions (or
     None); kcut: cutoff wavenumber (real time only) or None."""
     kind_code = _lib.IMAG if kind == "imaginary" else _lib.REAL
+    ctx = ctx or _lib.default_context()
+    # pass variants: tuned once per context and shape, then the context default
+    tune.ensure_tuned(n, precision_code(precision), kind_code, ctx)
     solve = _lib.Solve(n, precision_code(precision), kind_code, dt, beta, x_array, k_array, kcut=kcut, ctx=ctx)
     plan = plan_potential(static, dynamic if kind_code == _lib.REAL else None)
     for target, program in plan.static.items():

@@ -24,8 +24,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib
-from .engine import make_solve
+from . import _lib, tune
+from .engine import make_solve, precision_code
 from .grids import get_grids
 
 
@@ -148,12 +148,25 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
     if len({(r.n, r.length) for r in reqs}) != 1:
         raise ValueError("a sweep batches one grid; group requests by side and box first")
     phases = [precision] + ([polish] if polish else [])
+    # pass variants tuned for this batch shape (many members per launch)
+    batch_variants = {}
+    if tune.enabled():
+        for prec in phases:
+            t = tune.ensure_tuned(reqs[0].n, precision_code(prec), _lib.IMAG, ctx=ctx, batch=min(len(reqs), 64))
+            batch_variants[prec] = t.choice if t else None
+
+    def new_member(req, prec):
+        s = _new_solve(req, prec, ctx)
+        if batch_variants.get(prec):
+            s.set_variants(batch_variants[prec])
+        return s
+
     results = [Result() for _ in reqs]
     switches = [Switch(r.tol, plateau_checks) for r in reqs]
     phase_of = [0] * len(reqs)
     solves = []
     for req in reqs:
-        s = _new_solve(req, precision, ctx)
+        s = new_member(req, precision)
         s.gaussian(req.sigma)
         solves.append(s)
     active = list(range(len(reqs)))
@@ -194,7 +207,7 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
             for ph in [p for p, (m, _) in batches.items() if i in m]:
                 batches.pop(ph)[1].close()
             phase_of[i] += 1
-            s = _new_solve(reqs[i], phases[phase_of[i]], ctx)
+            s = new_member(reqs[i], phases[phase_of[i]])
             s.copy_from(solves[i])
             solves[i].close()
             solves[i] = s

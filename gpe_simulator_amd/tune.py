@@ -1,0 +1,134 @@
+"""Autotuner for the pass tile variants.
+
+Every pass kind of a step (z_first, z_mid, z_last, y_fwd, y_inv, x_kin) reads
+and writes the field in the same canonical layout, so each kind can run any of
+the compiled tile variants of its (side, precision): threads per pencil,
+pencils per block and the waves per SIMD its register budget allows
+(gpe_shapes.h).  The tuner times each variant on scratch solves of the real
+shape (alone, or as a batch of `batch` members, since small grids behave
+differently when many solves share a launch), with HIP events around every
+pass, and keeps the fastest variant per kind.  The choice becomes the
+context's default for new solves of that (side, precision, kind).
+
+Results are cached on the context; variants a caller set on the context
+explicitly are never replaced.  Set the environment variable
+GPE_HIP_AUTOTUNE=0 to keep variant 0 everywhere.
+"""
+
+from __future__ import annotations
+
+import os
+import threading
+from dataclasses import dataclass, field
+
+from . import _lib
+from .grids import get_grids
+
+STEP_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin")
+_lock = threading.Lock()
+
+
+@dataclass
+class Tuning:
+    n: int
+    precision: int
+    kind: int
+    batch: int
+    variants: list                       # [(tp, pencils per block, waves)]
+    choice: list                         # variant per pass kind (PASS_KINDS order)
+    timings: dict = field(default_factory=dict)   # variant -> {kind: avg us}
+
+    def report(self):
+        lines = [f"n={self.n} precision={'c64' if self.precision == _lib.C64 else 'c128'} "
+                 f"kind={'imaginary' if self.kind == _lib.IMAG else 'real'} batch={self.batch}"]
+        for v, geo in enumerate(self.variants):
+            t = self.timings.get(v, {})
+            lines.append(f"  v{v} tp={geo[0]} b={geo[1]} waves={geo[2]}: " +
+                         ", ".join(f"{k} {t[k]:.2f}" for k in STEP_KINDS if k in t))
+        lines.append("  choice: " + ", ".join(f"{k}=v{self.choice[_lib.PASS_KINDS.index(k)]}" for k in STEP_KINDS))
+        return "\n".join(lines)
+
+
+def enabled():
+    return os.environ.get("GPE_HIP_AUTOTUNE", "1") != "0"
+
+
+def _scratch(n, precision, kind, ctx):
+    length = 20.0
+    x, k = get_grids(n, length)
+    s = _lib.Solve(n, precision, kind, 5e-4, 100.0, x, k, ctx=ctx)
+    from .expressions import plan_potential
+    for target, program in plan_potential("x**2 + y**2 + z**2", None).static.items():
+        s.potential_static(target, program)
+    s.gaussian(1.5)
+    return s
+
+
+def measure(n, precision, kind, steps=8, batch=1, rounds=2, ctx=None):
+    """Average µs per launch of every step pass kind, for every variant:
+    {variant: {kind: us}} (the minimum over `rounds` timed runs)."""
+    ctx = ctx or _lib.default_context()
+    variants = _lib.pass_variants(n, precision)
+    solves = [_scratch(n, precision, kind, ctx) for _ in range(batch)]
+    group = _lib.Batch(solves, ctx=ctx) if batch > 1 else None
+    out = {}
+    try:
+        for v in range(len(variants)):
+            for s in solves:
+                s.set_variants([v] * len(_lib.PASS_KINDS))
+            best = {}
+            for _ in range(rounds):
+                if group is not None:
+                    group.step(2, graph=False)
+                    timing = group.step_timed(steps)
+                else:
+                    solves[0].step(2)
+                    timing = solves[0].step_timed(steps)
+                for name, (ms, count) in timing.items():
+                    if count and name in STEP_KINDS:
+                        us = ms / count * 1e3
+                        best[name] = min(best.get(name, us), us)
+            out[v] = best
+    finally:
+        if group is not None:
+            group.close()
+        for s in solves:
+            s.close()
+    return variants, out
+
+
+def autotune(n, precision, kind, steps=8, batch=1, rounds=2, ctx=None):
+    """Time every variant and return the fastest per pass kind (a Tuning)."""
+    precision = int(precision)
+    kind = int(kind)
+    variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx)
+    choice = [0] * len(_lib.PASS_KINDS)
+    for name in STEP_KINDS:
+        measured = [(timings[v][name], v) for v in timings if name in timings[v]]
+        if measured:
+            choice[_lib.PASS_KINDS.index(name)] = min(measured)[1]
+    return Tuning(n=n, precision=precision, kind=kind, batch=batch, variants=variants, choice=choice,
+                  timings=timings)
+
+
+def ensure_tuned(n, precision, kind, ctx=None, batch=1):
+    """Tune (side, precision, kind) once per context and, for solo tuning,
+    make the choice the context's default for new solves unless the caller
+    has set that default.  Returns the Tuning, or None when tuning is
+    disabled, the side has a single variant or the default was set."""
+    if not enabled():
+        return None
+    ctx = ctx or _lib.default_context()
+    n, precision, kind = int(n), int(precision), int(kind)
+    key = (n, precision, kind, int(batch))
+    with _lock:
+        if key in ctx.tunings:
+            return ctx.tunings[key]
+        if len(_lib.pass_variants(n, precision)) < 2 or (batch == 1 and key[:3] in ctx.variant_defaults):
+            ctx.tunings[key] = None
+            return None
+        result = autotune(n, precision, kind, batch=batch, ctx=ctx)
+        if batch == 1:
+            ctx.set_variants(n, precision, kind, result.choice)
+        ctx.tunings[key] = result
+        return result

@@ -52,6 +52,25 @@ int gpe_ctx_destroy(gpe_ctx* ctx);
 /* Free / total device memory in bytes (serving admission). */
 int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
+/* Pass tile variants.  A step is a sequence of passes (kinds 0..GPE_PASS_KINDS-1:
+ * z_first, z_mid, z_last, y_fwd, y_inv, x_kin, z_diag, x_diag, z_fwd, y_lz,
+ * x_lz); each pass kind of a solve runs one of the compiled tile variants of
+ * its (side, precision), all producing the same transform.  An autotuner
+ * times the variants and sets the fastest per kind.  No reference counterpart:
+ * the reference's FFT plans are cuFFT's (gpe/make_propagator.py:50,52). */
+enum { GPE_PASS_KINDS = 11 };
+/* Number of compiled variants for (side, precision); 1 for sides without a
+ * fast transform. */
+int gpe_pass_variants(int32_t n, int32_t precision, int32_t* count);
+/* Geometry of a variant: threads per pencil, pencils per block, and the waves
+ * per SIMD its register budget allows. */
+int gpe_pass_variant_info(int32_t n, int32_t precision, int32_t variant, int32_t* threads_per_pencil,
+                          int32_t* pencils_per_block, int32_t* waves);
+/* Variants a context gives new solves of (side, precision, kind); `variants`
+ * has GPE_PASS_KINDS entries. */
+int gpe_ctx_set_variants(gpe_ctx* ctx, int32_t n, int32_t precision, int32_t kind, const int32_t* variants,
+                         int32_t nkinds);
+
 /* Solve description.  Replaces what make_real_propagator / make_imag_propagator
  * (gpe/make_propagator.py:5, :106) read from `stage` (stage.beta, stage.dt,
  * stage.N_grids, stage.x_array, stage.k_array, stage.k_cut_params;
@@ -77,6 +96,10 @@ int gpe_solve_destroy(gpe_solve* solve);
 /* Device bytes a solve of this description holds (psi + tables), and the
  * extra bytes the diagnostics workspace adds when first used. */
 int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size_t* diag_bytes);
+/* Pass variant per kind of one solve (GPE_PASS_KINDS entries each).  Batch
+ * members must use the same variants. */
+int gpe_solve_set_variants(gpe_solve* solve, const int32_t* variants, int32_t nkinds);
+int gpe_solve_get_variants(gpe_solve* solve, int32_t* variants, int32_t nkinds);
 
 /* Copy elements [offset, offset+count) of psi (flat C-order index) from / to
  * a host complex128 buffer (interleaved re, im).  complex64 solves convert
