@@ -153,11 +153,11 @@ class Server:
             groups = {}
             for slot in active:
                 groups.setdefault((slot.req.n, slot.precision, slot.req.kind), []).append(slot)
-            launched, tick0 = [], time.perf_counter()
-            for members in groups.values():
-                batch = self._batch_for(members)
+            tick0 = time.perf_counter()
+            # every batch is built (device allocation) before any group runs
+            launched = [self._batch_for(members) for members in groups.values()]
+            for batch in launched:
                 batch.launch(self.check_every, graph=self.graph)
-                launched.append(batch)
             for batch in launched:
                 batch.wait()
             if not realtime:

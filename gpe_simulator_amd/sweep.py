@@ -63,6 +63,29 @@ def mu_from_norms(norms, dt):
     return -np.log(norms) / (2.0 * dt)
 
 
+class FieldFloor:
+    """Where a complex64 field stops relaxing, as a complex128 |dmu|.
+
+    complex64 rounding keeps re-exciting the slow modes, so a complex64 field
+    settles at a level that the complex128 estimate reads as |dmu| ~ F, far
+    below the complex64 estimate's own floor.  It is measured on the first
+    members to switch, all switched late (once their fitted decay reaches
+    tol): the complex128 |dmu| at their second check after the switch is
+    that level.  The median of `samples` such readings is used, and no later
+    reading (from members switched at F itself) feeds back into it."""
+
+    def __init__(self, samples=4):
+        self.samples, self.readings = samples, []
+
+    @property
+    def value(self):
+        return float(np.median(self.readings)) if len(self.readings) >= self.samples else None
+
+    def add(self, dmu):
+        if len(self.readings) < self.samples:
+            self.readings.append(dmu)
+
+
 class Switch:
     """When to move a complex64 imaginary-time solve to complex128.
 
@@ -72,39 +95,57 @@ class Switch:
     dtau = 5e-4), while the field itself keeps relaxing below that floor.
     Once the floor shows (no new minimum in `plateau` checks), the clean
     part of the history (|dmu| > 30x the floor) is fitted by a line in
-    log|dmu| and the solve stays in complex64 until the step where the
-    line reaches `tol`; the complex128 polish then only has to confirm
-    |dmu| < tol with its own, noise-free estimate.  Without a usable fit the
-    switch happens at once."""
+    log|dmu|.  The solve stays in complex64 until the line reaches `tol`, or
+    the complex64 field's own floor (a FieldFloor, once measured) if that is
+    higher: from there on complex64 steps no longer bring the field closer.
+    The complex128 polish then runs until its own, noise-free |dmu| < tol.
+    Without a usable fit the switch happens at once."""
 
-    def __init__(self, tol, plateau=10, clean_factor=30.0, fit_points=6):
+    def __init__(self, tol, plateau=10, clean_factor=30.0, fit_points=6, field_floor=None):
         self.tol, self.plateau, self.clean_factor, self.fit_points = tol, plateau, clean_factor, fit_points
+        self.field_floor = field_floor
         self.steps, self.dmu = [], []
-        self.target = None
+        self.fit = None                  # (slope, intercept) of log|dmu| against steps
+        self.detected = False
+        self.late = False                # switched where the line reached tol
 
     def update(self, steps, dmu):
         """Record the check at `steps`; True when the solve should switch."""
         self.steps.append(steps)
         self.dmu.append(dmu)
         if dmu < self.tol:
+            self.late = True
             return True
-        if self.target is None:
+        if not self.detected:
             p = self.plateau
             if len(self.dmu) <= p or min(self.dmu[-p:]) < min(self.dmu[:-p]):
                 return False
-            self.target = self.predict()
-        return steps >= self.target
+            self.detected = True
+            self.fit = self.predict()
+            if self.fit is None:
+                return True
+        floor = self.field_floor.value if self.field_floor is not None else None
+        level = max(self.tol, floor) if floor is not None else self.tol
+        self.late = floor is None
+        return steps >= self.step_at(level)
+
+    def step_at(self, level):
+        slope, icpt = self.fit
+        return int(np.ceil((np.log(level) - icpt) / slope))
+
+    @property
+    def target(self):
+        """Step where the fitted line reaches tol (None before the fit)."""
+        return self.step_at(self.tol) if self.fit else None
 
     def predict(self):
         floor = float(np.median(self.dmu[-self.plateau:]))
         clean = [(s, d) for s, d in zip(self.steps, self.dmu) if d > self.clean_factor * floor][-self.fit_points:]
         if len(clean) < 3:
-            return self.steps[-1]
+            return None
         s, d = np.array(clean, dtype=float).T
         slope, icpt = np.polyfit(s, np.log(d), 1)
-        if not slope < 0:
-            return self.steps[-1]
-        return int(np.ceil((np.log(self.tol) - icpt) / slope))
+        return (slope, icpt) if slope < 0 else None
 
 
 def _new_solve(req, precision, ctx=None):
@@ -162,8 +203,10 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
         return s
 
     results = [Result() for _ in reqs]
-    switches = [Switch(r.tol, plateau_checks) for r in reqs]
+    field_floor = FieldFloor()
+    switches = [Switch(r.tol, plateau_checks, field_floor=field_floor) for r in reqs]
     phase_of = [0] * len(reqs)
+    polish_checks = [0] * len(reqs)
     solves = []
     for req in reqs:
         s = new_member(req, precision)
@@ -171,18 +214,26 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
         solves.append(s)
     active = list(range(len(reqs)))
     batches = {}                       # phase -> (member tuple, Batch)
+    rounds_of = {}                     # phase -> rounds its current batch has run
     while active:
-        launched = []
-        for ph in range(len(phases)):
-            members = tuple(i for i in active if phase_of[i] == ph)
-            if not members:
-                continue
-            if ph not in batches or batches[ph][0] != members:
-                if ph in batches:
-                    batches.pop(ph)[1].close()
+        groups = {ph: tuple(i for i in active if phase_of[i] == ph) for ph in range(len(phases))}
+        # Retire stale batches before anything is launched: freeing device
+        # memory waits for the whole device, so a batch closed while another
+        # group runs would serialise the groups.
+        for ph in list(batches):
+            if batches[ph][0] != groups[ph]:
+                batches.pop(ph)[1].close()
+        launched = [(ph, members) for ph, members in groups.items() if members]
+        for ph, members in launched:
+            if ph not in batches:
                 batches[ph] = (members, _lib.Batch([solves[i] for i in members], ctx=ctx))
-            batches[ph][1].launch(check_every, graph=graph)
-            launched.append((ph, members))
+                rounds_of[ph] = 0
+        # A batch is captured into a graph once it has outlived one round:
+        # capturing for a membership that changes again next round costs more
+        # (graph instantiation) than the launches it saves.
+        for ph, _ in launched:
+            batches[ph][1].launch(check_every, graph=graph and rounds_of[ph] > 0)
+            rounds_of[ph] += 1
         for ph, _ in launched:
             batches[ph][1].wait()
         still, promote = [], []
@@ -200,8 +251,12 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
                     continue
                 if bulk and switches[i].update(res.steps, res.dmu):
                     promote.append(i)
-                elif not bulk and res.dmu < req.tol:
-                    continue
+                elif not bulk:
+                    polish_checks[i] += 1
+                    if polish_checks[i] == 2 and switches[i].late:
+                        field_floor.add(res.dmu)
+                    if res.dmu < req.tol:
+                        continue
                 still.append(i)
         for i in promote:
             for ph in [p for p, (m, _) in batches.items() if i in m]:

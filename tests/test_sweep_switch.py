@@ -3,7 +3,7 @@
 import numpy as np
 import pytest
 
-from gpe_simulator_amd.sweep import Switch, mu_from_norms
+from gpe_simulator_amd.sweep import FieldFloor, Switch, mu_from_norms
 
 
 def feed(switch, steps, dmus):
@@ -52,10 +52,39 @@ def test_flat_history_switches_when_the_plateau_shows():
     dmus = 1e-6 * rng.uniform(0.5, 1.5, steps.size)
     switch = Switch(1e-9, plateau=10)
     at = feed(switch, steps, dmus)
-    assert at is not None and at == switch.target
+    assert at is not None and switch.target is None
+    # the first check at which no new minimum appeared in the last 10
+    first = next(j for j in range(11, steps.size) if min(dmus[j - 9:j + 1]) >= min(dmus[:j - 9]))
+    assert at == steps[first]
 
 
 def test_mu_from_norms_inverts_norm_decay():
     dt, mu = 5e-4, 7.25
     norms = np.full(10, np.exp(-2 * mu * dt))
     np.testing.assert_allclose(mu_from_norms(norms, dt), mu, rtol=1e-13)
+
+
+def test_measured_field_floor_moves_the_switch_earlier():
+    steps = np.arange(50, 20001, 50)
+    rate, amp, floor, tol = np.log(4) / 400, 1e-1, 1e-6, 2e-9
+    dmus, _ = decaying(rate, amp, floor, steps)
+    field = FieldFloor(samples=2)
+    late = Switch(tol, field_floor=field)
+    feed(late, steps, dmus)
+    assert late.late
+    field.add(1e-8)
+    assert field.value is None
+    field.add(3e-8)
+    field.add(1e-3)                   # later readings do not feed back
+    assert field.value == pytest.approx(2e-8)
+    early = Switch(tol, field_floor=field)
+    at = feed(early, steps, dmus)
+    assert not early.late
+    # the fitted line reaches the measured floor after the plateau is seen
+    assert at == steps[np.searchsorted(steps, np.log(amp / 2e-8) / rate)]
+    assert at < late.target
+
+
+def test_mu_below_tol_marks_the_switch_late():
+    switch = Switch(1e-9, field_floor=FieldFloor())
+    assert switch.update(50, 1e-10) and switch.late
