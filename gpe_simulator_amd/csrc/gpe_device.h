@@ -269,7 +269,31 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> c) {
 __device__ __attribute__((noinline)) void wide_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
 __device__ __attribute__((noinline)) void wide_sincos(double x, double* s, double* c) { sincos(x, s, c); }
 
+// Polynomials on |r| <= pi/4 (Taylor to r^9 / r^10 in float, r^19 / r^18 in
+// double).
+__device__ __forceinline__ void sincos_poly(float r, float* s, float* c) {
+  const float r2 = r * r;
+  float ps = fmaf(r2, float(1.0 / 362880.0), float(-1.0 / 5040.0));
+  ps = fmaf(r2, ps, float(1.0 / 120.0));
+  ps = fmaf(r2, ps, float(-1.0 / 6.0));
+  *s = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, float(-1.0 / 3628800.0), float(1.0 / 40320.0));
+  pc = fmaf(r2, pc, float(-1.0 / 720.0));
+  pc = fmaf(r2, pc, float(1.0 / 24.0));
+  pc = fmaf(r2, pc, -0.5f);
+  *c = fmaf(r2, pc, 1.0f);
+}
+
+// Cody-Waite reduction by pi/2 (three-part constant), then the polynomials.
+// For |x| < 0.78 the reduction leaves r = x exactly (q = rint(x*2/pi) = 0 and
+// fma(0, ., x) = x) and the quadrant fix-up is the identity, so the short
+// path returns the same bits without doing either; split-step phases are
+// usually that small.
 __device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
+  if (fabsf(x) < 0.78f) {
+    sincos_poly(x, s, c);
+    return;
+  }
   if (!(fabsf(x) < 8192.0f)) {
     wide_sincos(x, s, c);
     return;
@@ -278,16 +302,8 @@ __device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
   float r = fmaf(q, -1.5707963705062866f, x);
   r = fmaf(q, 4.371138828673793e-08f, r);
   r = fmaf(q, 1.7151245100058819e-15f, r);
-  const float r2 = r * r;
-  float ps = fmaf(r2, float(1.0 / 362880.0), float(-1.0 / 5040.0));
-  ps = fmaf(r2, ps, float(1.0 / 120.0));
-  ps = fmaf(r2, ps, float(-1.0 / 6.0));
-  const float sn = fmaf(r * r2, ps, r);
-  float pc = fmaf(r2, float(-1.0 / 3628800.0), float(1.0 / 40320.0));
-  pc = fmaf(r2, pc, float(-1.0 / 720.0));
-  pc = fmaf(r2, pc, float(1.0 / 24.0));
-  pc = fmaf(r2, pc, -0.5f);
-  const float cs = fmaf(r2, pc, 1.0f);
+  float sn, cs;
+  sincos_poly(r, &sn, &cs);
   const int quad = int(q) & 3;
   const float a = (quad & 1) ? cs : sn;
   const float b = (quad & 1) ? sn : cs;
@@ -295,15 +311,7 @@ __device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
   *c = ((quad + 1) & 2) ? -b : b;
 }
 
-__device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
-  if (!(fabs(x) < 1.0e9)) {
-    wide_sincos(x, s, c);
-    return;
-  }
-  const double q = rint(x * 0.63661977236758134308);
-  double r = fma(q, -1.5707963267948966, x);
-  r = fma(q, -6.123233995736766e-17, r);
-  r = fma(q, 1.4973849048591698e-33, r);
+__device__ __forceinline__ void sincos_poly(double r, double* s, double* c) {
   const double r2 = r * r;
   double ps = fma(r2, -1.0 / 121645100408832000.0, 1.0 / 355687428096000.0);
   ps = fma(r2, ps, -1.0 / 1307674368000.0);
@@ -313,7 +321,7 @@ __device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
   ps = fma(r2, ps, -1.0 / 5040.0);
   ps = fma(r2, ps, 1.0 / 120.0);
   ps = fma(r2, ps, -1.0 / 6.0);
-  const double sn = fma(r * r2, ps, r);
+  *s = fma(r * r2, ps, r);
   double pc = fma(r2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
   pc = fma(r2, pc, 1.0 / 87178291200.0);
   pc = fma(r2, pc, -1.0 / 479001600.0);
@@ -322,7 +330,24 @@ __device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
   pc = fma(r2, pc, 1.0 / 720.0);
   pc = fma(r2, pc, -1.0 / 24.0);
   pc = fma(r2, pc, 0.5);
-  const double cs = fma(r2, -pc, 1.0);
+  *c = fma(r2, -pc, 1.0);
+}
+
+__device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
+  if (fabs(x) < 0.78) {
+    sincos_poly(x, s, c);
+    return;
+  }
+  if (!(fabs(x) < 1.0e9)) {
+    wide_sincos(x, s, c);
+    return;
+  }
+  const double q = rint(x * 0.63661977236758134308);
+  double r = fma(q, -1.5707963267948966, x);
+  r = fma(q, -6.123233995736766e-17, r);
+  r = fma(q, 1.4973849048591698e-33, r);
+  double sn, cs;
+  sincos_poly(r, &sn, &cs);
   const int quad = int(q) & 3;
   const double a = (quad & 1) ? cs : sn;
   const double b = (quad & 1) ? sn : cs;

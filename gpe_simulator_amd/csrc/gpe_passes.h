@@ -110,10 +110,11 @@ struct FastTF {
   __device__ static __forceinline__ int pos(int t, int m) { return t + TP * m; }
   __device__ static __forceinline__ bool valid(int, int, int) { return true; }
 
-  // Twiddles W_N^{t*k1} staged once per block in LDS as twl[k1*TP + t].
-  static constexpr int TWL_ELEMS = M * TP;
+  // Twiddles W_N^{t*k1}, k1 >= 1 (k1 = 0 is the unit factor), staged once
+  // per block in LDS as twl[(k1-1)*TP + t].
+  static constexpr int TWL_ELEMS = (M - 1) * TP;
   __device__ static __forceinline__ void stage_twiddles(cpx<double>* twl, const cpx<double>* tw) {
-    for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) twl[e] = gload(tw + (e % TP) * (e / TP));
+    for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) twl[e] = gload(tw + (e % TP) * (e / TP + 1));
     __syncthreads();
   }
 
@@ -122,7 +123,7 @@ struct FastTF {
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
-      cpx<double> w = twl[k1 * TP + t];
+      cpx<double> w = twl[(k1 - 1) * TP + t];
       if constexpr (DIR > 0) w.y = -w.y;
       v[k1] = cmul_d(v[k1], w);
       // scheduling fence every 4 twiddles: the FP64 twiddles (4 registers

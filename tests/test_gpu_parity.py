@@ -133,3 +133,19 @@ def test_gaussian_initial_state(require_gpu):
     solve.gaussian(1.3)
     np.testing.assert_allclose(solve.to_numpy(), orc.gaussian(x, 1.3), rtol=1e-13, atol=1e-15)
     assert solve.norm() == pytest.approx(1.0, abs=1e-13)
+
+
+@pytest.mark.parametrize("precision,static,tol", [
+    # half-step phases up to a few radians: Cody-Waite reduction path
+    ("c128", "40*(x*x + y*y + z*z)", 1e-11),
+    ("c64", "40*(x*x + y*y + z*z)", 2e-5),
+    # phases of ~1e4 rad and beyond: the wide-argument fallback (complex128;
+    # a float phase that large carries ~1e-3 rad of rounding by itself)
+    ("c128", "2e7 + 40*(x*x + y*y)", 1e-10),
+])
+def test_large_phases(precision, static, tol, require_gpu):
+    n, length = 18, 12.0
+    problem, solve = make(n, length, "real", 1e-3, 50.0, static, precision=precision)
+    psi0 = random_state(n, length, seed=3)
+    got, expected = run_both(problem, solve, psi0, 3)
+    assert rel_l2(got, expected) < tol
