@@ -10,8 +10,9 @@ HBM before timing starts.
 
 Beside it, the same line reports: the complex128 run of the same workload,
 per-kernel HIP-event timings, the oracle on the host as the CPU baseline, a
-short seeded serving stream (p50/p99 time-to-result) and the 64-solve 96^3
-batched ground-state sweep against sequential complex128 solves.
+short seeded serving stream (p50/p99 time-to-result), the 64-solve 96^3
+batched ground-state sweep against sequential complex128 solves, and a
+16-speed stirring sweep at 128^3 batched against the same runs in turn.
 
 Prints ONE JSON line.  `value` = grid-point-steps/s of the complex64 run.
 """
@@ -138,6 +139,30 @@ def sweep(count, every=1, seed=0):
             "mu_rel_diff_vs_sequential": max(abs(res[i].mu - s.mu) / abs(s.mu) for i, s in seq.items())}
 
 
+def stirring(count, n=128, steps=500):
+    """`count` stirring runs at different stirrer speeds (n^3, g = 500,
+    beam amplitude 5 on radius 3, dt' = 5e-4, complex64), from one seeded
+    state, batched against the same runs one after another."""
+    from gpe_simulator_amd.sweep import stir_batched, stir_sequential, stirrer_runs
+    from gpe_simulator_amd.grids import get_grids
+    runs = stirrer_runs(np.linspace(0.1, 1.5, count), n=n, length=24.0, g=500.0, steps=steps)
+    x, _ = get_grids(n, 24.0)
+    rng = np.random.default_rng(1)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    psi0 = np.exp(-(X * X + Y * Y + Z * Z) / 8).astype(np.complex128)
+    psi0 += 1e-3 * (rng.standard_normal(psi0.shape) + 1j * rng.standard_normal(psi0.shape))
+    stir_batched(runs[:2], psi0)                       # tuning and first-use setup outside the timing
+    t0 = time.perf_counter()
+    batched = stir_batched(runs, psi0)
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    seq = [stir_sequential(r, psi0) for r in runs]
+    ts = time.perf_counter() - t0
+    return {"runs": count, "grid": n, "steps": steps, "batched_s": tb, "sequential_s": ts, "speedup": ts / tb,
+            "identical_to_sequential": all(a == b for a, b in zip(batched, seq)),
+            "Lz_range": [min(r["Lz"] for r in batched), max(r["Lz"] for r in batched)]}
+
+
 def cpu_baseline(psi0, samples):
     """The FP64 oracle (numpy.fft, single thread) on a bounded sample."""
     from oracle import gpe_oracle as orc
@@ -163,6 +188,7 @@ def main(argv=None):
     parser.add_argument("--skip-c128", action="store_true")
     parser.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving stream")
     parser.add_argument("--sweep-count", type=int, default=64, help="0 skips the batched sweep")
+    parser.add_argument("--stir-count", type=int, default=16, help="0 skips the stirring-speed sweep")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
@@ -221,6 +247,11 @@ def main(argv=None):
         log(f"batched sweep: {args.sweep_count} ground states at 96^3")
         swept = sweep(args.sweep_count)
 
+    stirred = None
+    if args.stir_count > 0:
+        log(f"stirring sweep: {args.stir_count} stirrer speeds at 128^3")
+        stirred = stirring(args.stir_count)
+
     step_bytes = PASSES_PER_STEP * 2 * 8
     line = {
         "metric": "3D SSFM grid-pt-steps/s at 256^3 c64 (% HBM roofline) vs c128; p50/p99 solve ms",
@@ -250,6 +281,7 @@ def main(argv=None):
         "p99_solve_ms": served["p99_ms"] if served else None,
         "serving": served,
         "sweep": swept,
+        "stirring_sweep": stirred,
     }
     print(json.dumps(line), flush=True)
     return 0

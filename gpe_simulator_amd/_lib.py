@@ -32,7 +32,7 @@ EXPORTS = (
     "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
-    "gpe_solve_get_variants",
+    "gpe_solve_get_variants", "gpe_batch_step_times",
 )
 
 
@@ -99,6 +99,7 @@ def _declare(lib):
         "gpe_ctx_set_variants": (c_int, [vp, i32, i32, i32, POINTER(i32), i32]),
         "gpe_solve_set_variants": (c_int, [vp, POINTER(i32), i32]),
         "gpe_solve_get_variants": (c_int, [vp, POINTER(i32), i32]),
+        "gpe_batch_step_times": (c_int, [vp, i64, dp, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -375,8 +376,23 @@ class Batch:
         check(lib().gpe_batch_create(self.ctx.handle, arr, len(self.solves), byref(handle)))
         self.handle = handle
 
-    def step(self, nsteps, graph=True):
-        check(lib().gpe_batch_step(self.handle, int(nsteps), 1 if graph else 0))
+    def step(self, nsteps, graph=True, t_mid=None):
+        """Advance every member nsteps.  Members with time-dependent programs
+        need t_mid: shape (nsteps,) shared by all members, or
+        (members, nsteps) per member; such batches are not graph-captured."""
+        if t_mid is None:
+            check(lib().gpe_batch_step(self.handle, int(nsteps), 1 if graph else 0))
+            return
+        tm = np.ascontiguousarray(t_mid, dtype=np.float64)
+        if tm.ndim == 1:
+            if tm.size < nsteps:
+                raise GpeInvalid(GPE_ERR_INVALID, "t_mid needs one time per step")
+            stride = 0
+        else:
+            if tm.shape[0] != len(self.solves) or tm.shape[1] < nsteps:
+                raise GpeInvalid(GPE_ERR_INVALID, "t_mid must be (members, nsteps)")
+            stride = tm.shape[1]
+        check(lib().gpe_batch_step_times(self.handle, int(nsteps), dptr(tm), int(stride)))
 
     def launch(self, nsteps, graph=True):
         check(lib().gpe_batch_launch(self.handle, int(nsteps), 1 if graph else 0))

@@ -413,14 +413,18 @@ bool any_dynamic(gpe_solve* s) { return s->dyn[0].active || s->dyn[1].active || 
 
 // Evaluate every active time-dependent program at times t[0..nslots) into
 // slot tables (diag == false) or the single diagnostic slot (diag == true).
-int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag) {
+// Evaluates the time-dependent tables at times t[0..nslots) into the step
+// slots (or the diagnostic slot), on `stream` (the solve's own by default;
+// a batch uses its stream so the tables stay ordered with its passes).
+int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream_t stream = nullptr) {
+  hipStream_t st = stream ? stream : s->st;
   const int f = s->tlist_flip;
   s->tlist_flip ^= 1;
   HIPCHK(hipEventSynchronize(s->tlist_ev[f]));
   std::memcpy(s->h_tlist[f], t, sizeof(double) * nslots);
   double* dst_t = s->d_tlist;
-  HIPCHK(hipMemcpyAsync(dst_t, s->h_tlist[f], sizeof(double) * nslots, hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipEventRecord(s->tlist_ev[f], s->st));
+  HIPCHK(hipMemcpyAsync(dst_t, s->h_tlist[f], sizeof(double) * nslots, hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(s->tlist_ev[f], st));
   for (int g = 0; g < 3; ++g) {
     if (!s->dyn[g].active) continue;
     const int target = g == 2 ? GPE_TGT_VOL : g;
@@ -429,10 +433,10 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag) {
     void* out = diag ? s->g_tab[g] : s->d_tab[g];
     if (s->prec == GPE_C64)
       launch_vm_dynamic<float>(s->dyn[g].code, s->dyn[g].ncode, s->dyn[g].consts, target, s->n, s->xa, size,
-                               master, dst_t, nslots, out, s->st);
+                               master, dst_t, nslots, out, st);
     else
       launch_vm_dynamic<double>(s->dyn[g].code, s->dyn[g].ncode, s->dyn[g].consts, target, s->n, s->xa, size,
-                                master, dst_t, nslots, out, s->st);
+                                master, dst_t, nslots, out, st);
     HIPCHK(hipGetLastError());
   }
   return GPE_OK;
@@ -1091,7 +1095,6 @@ int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_
     if (!s || s->ctx != ctx) return fail(GPE_ERR_INVALID, "batch solve missing or on another context");
     if (s->n != solves[0]->n || s->prec != solves[0]->prec || s->kind != solves[0]->kind)
       return fail(GPE_ERR_INVALID, "a batch needs solves of one side, precision and kind");
-    if (any_dynamic(s)) return fail(GPE_ERR_INVALID, "batched solves cannot carry time-dependent programs");
     for (int j = 0; j < i; ++j)
       if (solves[j] == s) return fail(GPE_ERR_INVALID, "a solve appears twice in the batch");
   }
@@ -1181,6 +1184,9 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   if (!b) return fail(GPE_ERR_INVALID, "null batch");
   if (nsteps < 0) return fail(GPE_ERR_INVALID, "nsteps must be >= 0");
   if (nsteps == 0) return GPE_OK;
+  for (gpe_solve* s : b->solves)
+    if (any_dynamic(s))
+      return fail(GPE_ERR_INVALID, "a batch member carries a time-dependent program: step with gpe_batch_step_times");
   HIPCHK(hipSetDevice(b->ctx->device));
   bool pre = false;
   CHK(batch_prepare(b, nsteps, &pre));
@@ -1228,6 +1234,50 @@ int gpe_batch_wait(gpe_batch* b) {
 int gpe_batch_step(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   CHK(gpe_batch_launch(b, nsteps, use_graph));
   return gpe_batch_wait(b);
+}
+
+int gpe_batch_step_times(gpe_batch* b, int64_t nsteps, const double* t_mid, int64_t stride) {
+  if (!b) return fail(GPE_ERR_INVALID, "null batch");
+  if (nsteps < 0 || stride < 0) return fail(GPE_ERR_INVALID, "nsteps and stride must be >= 0");
+  if (nsteps == 0) return GPE_OK;
+  bool dyn = false;
+  int chunk = 1 << 30;
+  for (gpe_solve* s : b->solves)
+    if (any_dynamic(s)) {
+      dyn = true;
+      chunk = std::min(chunk, s->chunk);
+    }
+  if (!dyn) return gpe_batch_step(b, nsteps, 1);
+  if (!t_mid) return fail(GPE_ERR_INVALID, "t_mid is required when a member carries a time-dependent program");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  bool pre = false;
+  CHK(batch_prepare(b, nsteps, &pre));
+  // Same pass sequence as a single real-time solve (step_real); before each
+  // chunk every time-dependent member evaluates its own slots on the batch
+  // stream, so the tables are ordered with the passes that read them.
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t cs = std::min<int64_t>(chunk, nsteps - done);
+    const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
+    for (size_t i = 0; i < b->solves.size(); ++i)
+      if (any_dynamic(b->solves[i])) CHK(eval_dynamic(b->solves[i], t_mid + i * stride + done, nslots, false, b->st));
+    for (int64_t q = 0; q < cs; ++q) {
+      const int64_t step = done + q;
+      const int sa = (int)q, sb = (int)q + 1;
+      if (step == 0) CHK(batch_pass(b, K_Z_FIRST, sa, 0, pre ? F_PRESCALE : 0, 0));
+      CHK(batch_pass(b, K_Y_FWD, 0, 0, 0, 0));
+      CHK(batch_pass(b, K_X_KIN, 0, 0, 0, 0));
+      CHK(batch_pass(b, K_Y_INV, 0, 0, 0, 0));
+      if (step == nsteps - 1)
+        CHK(batch_pass(b, K_Z_LAST, sa, 0, 0, 0));
+      else
+        CHK(batch_pass(b, K_Z_MID, sa, sb, 0, 0));
+    }
+    done += cs;
+  }
+  batch_finish(b, nsteps);
+  HIPCHK(hipStreamSynchronize(b->st));
+  return GPE_OK;
 }
 
 int gpe_batch_step_timed(gpe_batch* b, int64_t nsteps, double* ms_per_kind, int64_t* launches_per_kind) {

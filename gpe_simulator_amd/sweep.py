@@ -1,12 +1,17 @@
-"""Batched imaginary-time ground states (parameter sweeps).
+"""Batched parameter sweeps: imaginary-time ground states and stirring runs.
 
-Many independent solves of one grid side advance together: every pass is one
-launch over all members (gpe_batch_*), and the check_every-step launch
-sequence is replayed from a captured hipGraph.  The bulk of the relaxation
-runs with complex64 fields and FP64 reductions; when its Switch says so, a
-member is copied into a complex128 field and polished, still batched, until
-|dmu| < tol.  The complex64 and complex128 groups are launched on their own
-streams each round and overlap on the device.
+Ground states: many independent solves advance together, one batch per
+(phase, grid side): every pass is one launch over all members of a batch
+(gpe_batch_*), and the check_every-step launch sequence is replayed from a
+captured hipGraph.  The bulk of the relaxation runs with complex64 fields
+and FP64 reductions; when its Switch says so, a member is copied into a
+complex128 field and polished, still batched, until |dmu| < tol.  All
+groups are launched on their own streams each round and overlap on the
+device.
+
+Stirring runs (stir_batched): real-time runs whose stirrers differ (speed,
+radius, amplitude) advance as one batch; each member evaluates its own
+time-dependent beam before every chunk of steps.
 
 Convergence uses the per-step norm the engine records before renormalising:
 starting from a normalised state, one imaginary-time step of length dtau
@@ -183,28 +188,31 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
 
     With `polish` set, each member runs in `precision` until its Switch says
     so, then in `polish` until |dmu| < tol; without it, in `precision` only.
-    Every round advances each group by check_every steps."""
+    Requests may have different grid sides: members are grouped by (phase,
+    side), each group is one batch on its own stream, and every round
+    advances every group by check_every steps."""
     t0 = time.perf_counter()
     ctx = ctx or _lib.default_context()
-    if len({(r.n, r.length) for r in reqs}) != 1:
-        raise ValueError("a sweep batches one grid; group requests by side and box first")
     phases = [precision] + ([polish] if polish else [])
-    # pass variants tuned for this batch shape (many members per launch)
+    sides = sorted({r.n for r in reqs})
+    # pass variants tuned for each batch shape (many members per launch)
     batch_variants = {}
     if tune.enabled():
-        for prec in phases:
-            t = tune.ensure_tuned(reqs[0].n, precision_code(prec), _lib.IMAG, ctx=ctx, batch=min(len(reqs), 64))
-            batch_variants[prec] = t.choice if t else None
+        for n in sides:
+            count = sum(r.n == n for r in reqs)
+            for prec in phases:
+                t = tune.ensure_tuned(n, precision_code(prec), _lib.IMAG, ctx=ctx, batch=min(count, 64))
+                batch_variants[(prec, n)] = t.choice if t else None
 
     def new_member(req, prec):
         s = _new_solve(req, prec, ctx)
-        if batch_variants.get(prec):
-            s.set_variants(batch_variants[prec])
+        if batch_variants.get((prec, req.n)):
+            s.set_variants(batch_variants[(prec, req.n)])
         return s
 
     results = [Result() for _ in reqs]
-    field_floor = FieldFloor()
-    switches = [Switch(r.tol, plateau_checks, field_floor=field_floor) for r in reqs]
+    field_floor = {n: FieldFloor() for n in sides}
+    switches = [Switch(r.tol, plateau_checks, field_floor=field_floor[r.n]) for r in reqs]
     phase_of = [0] * len(reqs)
     polish_checks = [0] * len(reqs)
     solves = []
@@ -213,31 +221,32 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
         s.gaussian(req.sigma)
         solves.append(s)
     active = list(range(len(reqs)))
-    batches = {}                       # phase -> (member tuple, Batch)
-    rounds_of = {}                     # phase -> rounds its current batch has run
+    batches = {}                       # (phase, side) -> (member tuple, Batch)
+    rounds_of = {}                     # (phase, side) -> rounds its current batch has run
+    keys = [(ph, n) for ph in range(len(phases)) for n in sides]
     while active:
-        groups = {ph: tuple(i for i in active if phase_of[i] == ph) for ph in range(len(phases))}
+        groups = {key: tuple(i for i in active if (phase_of[i], reqs[i].n) == key) for key in keys}
         # Retire stale batches before anything is launched: freeing device
         # memory waits for the whole device, so a batch closed while another
         # group runs would serialise the groups.
-        for ph in list(batches):
-            if batches[ph][0] != groups[ph]:
-                batches.pop(ph)[1].close()
-        launched = [(ph, members) for ph, members in groups.items() if members]
-        for ph, members in launched:
-            if ph not in batches:
-                batches[ph] = (members, _lib.Batch([solves[i] for i in members], ctx=ctx))
-                rounds_of[ph] = 0
+        for key in list(batches):
+            if batches[key][0] != groups[key]:
+                batches.pop(key)[1].close()
+        launched = [(key, members) for key, members in groups.items() if members]
+        for key, members in launched:
+            if key not in batches:
+                batches[key] = (members, _lib.Batch([solves[i] for i in members], ctx=ctx))
+                rounds_of[key] = 0
         # A batch is captured into a graph once it has outlived one round:
         # capturing for a membership that changes again next round costs more
         # (graph instantiation) than the launches it saves.
-        for ph, _ in launched:
-            batches[ph][1].launch(check_every, graph=graph and rounds_of[ph] > 0)
-            rounds_of[ph] += 1
-        for ph, _ in launched:
-            batches[ph][1].wait()
+        for key, _ in launched:
+            batches[key][1].launch(check_every, graph=graph and rounds_of[key] > 0)
+            rounds_of[key] += 1
+        for key, _ in launched:
+            batches[key][1].wait()
         still, promote = [], []
-        for ph, members in launched:
+        for (ph, _n), members in launched:
             bulk = ph + 1 < len(phases)
             for i in members:
                 req, res = reqs[i], results[i]
@@ -254,19 +263,20 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
                 elif not bulk:
                     polish_checks[i] += 1
                     if polish_checks[i] == 2 and switches[i].late:
-                        field_floor.add(res.dmu)
+                        field_floor[req.n].add(res.dmu)
                     if res.dmu < req.tol:
                         continue
                 still.append(i)
         for i in promote:
-            for ph in [p for p, (m, _) in batches.items() if i in m]:
-                batches.pop(ph)[1].close()
+            for key in [k for k, (m, _) in batches.items() if i in m]:
+                batches.pop(key)[1].close()
             phase_of[i] += 1
             s = new_member(reqs[i], phases[phase_of[i]])
             s.copy_from(solves[i])
             solves[i].close()
             solves[i] = s
-        active = [i for i in active if i in set(still)]
+        keep = set(still)
+        active = [i for i in active if i in keep]
     for _, b in batches.values():
         b.close()
     for i, req in enumerate(reqs):
@@ -288,3 +298,73 @@ def config_sweep(count=64, seed=0, n=96, length=20.0, dt_osc=1e-3, tol_osc=1e-9)
     sigma = rng.uniform(0.8, 1.5, count)
     return [GroundState(n=n, length=length, beta=2 * g[i], static=harmonic_expression(1, 1, wz[i]),
                         sigma=float(sigma[i]), dt=dt_osc / 2, tol=2 * tol_osc) for i in range(count)]
+
+
+# --------------------------------------------------------- stirring sweeps --
+@dataclass
+class StirRun:
+    """One real-time run with a moving stirrer (reference units)."""
+    n: int
+    length: float
+    beta: float
+    static: str
+    dynamic: str
+    dt: float
+    steps: int
+
+
+def stirrer_runs(omegas, n=128, length=24.0, g=500.0, amplitude=5.0, radius=3.0, width2=2.0, dt_osc=5e-4,
+                 steps=2000):
+    """Stirring runs at the stirrer angular speeds `omegas` (oscillator
+    units), otherwise identical: isotropic trap and a Gaussian beam
+    amplitude*exp(-d^2/width2) on a circle of `radius`.  Mapped to reference
+    units (V and beam doubled, beta = 2g, dt = dt'/2, Omega = 2 Omega')."""
+    from .units import harmonic_expression, stirrer_expression
+    return [StirRun(n=n, length=length, beta=2 * g, static=harmonic_expression(1, 1, 1),
+                    dynamic=stirrer_expression(amplitude, radius, omega, width2), dt=dt_osc / 2, steps=steps)
+            for omega in omegas]
+
+
+def _stir_solve(run, precision, ctx):
+    x, k = get_grids(run.n, run.length)
+    return make_solve(run.n, x, k, "real", run.dt, run.beta, static=run.static, dynamic=run.dynamic,
+                      precision=precision, ctx=ctx)
+
+
+def _stir_report(solve, run):
+    t = run.steps * run.dt
+    diag = solve.diagnostics(t, spectrum=False)
+    return {"E": diag["E"], "mu": diag["mu"], "norm": diag["norm"], "Lz": solve.angular_momentum()}
+
+
+def stir_batched(runs, psi0, precision="c64", ctx=None):
+    """Advance every run from the same initial field psi0 (host complex128)
+    in one batch (one launch per pass for all runs; each run evaluates its
+    own stirrer).  Returns one report (E, mu, norm, Lz at the end) per run."""
+    from .engine import midpoint_times
+    ctx = ctx or _lib.default_context()
+    if len({(r.n, r.length, r.steps) for r in runs}) != 1:
+        raise ValueError("a stirring batch shares one grid and step count")
+    solves = [_stir_solve(r, precision, ctx) for r in runs]
+    solves[0].upload(psi0)
+    for s in solves[1:]:
+        s.copy_from(solves[0])
+    batch = _lib.Batch(solves, ctx=ctx)
+    t_mid = np.stack([midpoint_times(0, r.steps, r.dt) for r in runs])
+    batch.step(runs[0].steps, t_mid=t_mid)
+    batch.close()
+    out = [_stir_report(s, r) for s, r in zip(solves, runs)]
+    for s in solves:
+        s.close()
+    return out
+
+
+def stir_sequential(run, psi0, precision="c64", ctx=None):
+    """The same run alone (one solve, one gpe_step call)."""
+    from .engine import midpoint_times
+    solve = _stir_solve(run, precision, ctx or _lib.default_context())
+    solve.upload(psi0)
+    solve.step(run.steps, midpoint_times(0, run.steps, run.dt))
+    out = _stir_report(solve, run)
+    solve.close()
+    return out

@@ -64,7 +64,7 @@ def _scratch(n, precision, kind, ctx):
     return s
 
 
-def measure(n, precision, kind, steps=8, batch=1, rounds=2, ctx=None):
+def measure(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
     """Average µs per launch of every step pass kind, for every variant:
     {variant: {kind: us}} (the minimum over `rounds` timed runs)."""
     ctx = ctx or _lib.default_context()
@@ -97,7 +97,7 @@ def measure(n, precision, kind, steps=8, batch=1, rounds=2, ctx=None):
     return variants, out
 
 
-def autotune(n, precision, kind, steps=8, batch=1, rounds=2, ctx=None):
+def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
     """Time every variant and return the fastest per pass kind (a Tuning)."""
     precision = int(precision)
     kind = int(kind)

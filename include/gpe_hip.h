@@ -169,18 +169,25 @@ int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts
  * Reference: get_L (gpe/helper.py:143-156). */
 int gpe_angular_momentum(gpe_solve* solve, double* out);
 
-/* Batches: equal-shape solves (same side, precision and kind, no
- * time-dependent program) advanced together, every pass one launch over all
- * members (the member index is the third grid dimension).  Each member keeps
- * its own psi, potential, beta, dt and reductions, so its result is the same
- * as when it runs alone.  With use_graph != 0 the nsteps-step launch sequence
- * is captured once in a hipGraph and replayed.  The batch borrows its
- * members: do not step a member alone while its batch is stepping.  No
- * reference counterpart (the reference runs one stage at a time,
- * gpe/main.py:30-38). */
+/* Batches: equal-shape solves (same side, precision and kind) advanced
+ * together, every pass one launch over all members (the member index is the
+ * third grid dimension).  Each member keeps its own psi, potentials (static
+ * and time-dependent), beta, dt, pass variants and reductions, so its result
+ * is the same as when it runs alone.  With use_graph != 0 the nsteps-step
+ * launch sequence is captured once in a hipGraph and replayed.  The batch
+ * borrows its members: do not step a member alone while its batch is
+ * stepping.  No reference counterpart (the reference runs one stage at a
+ * time, gpe/main.py:30-38). */
 int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_batch** out);
 int gpe_batch_destroy(gpe_batch* batch);
+/* Steps batches whose members carry no time-dependent program. */
 int gpe_batch_step(gpe_batch* batch, int64_t nsteps, int32_t use_graph);
+/* Steps a real-time batch whose members may carry time-dependent programs
+ * (e.g. stirrers of different speeds): member i takes its FP64 midpoint times
+ * from t_mid + i*stride (stride 0: all members share t_mid[0..nsteps)), as
+ * gpe_step does for one solve.  Every member's tables are evaluated on the
+ * batch's stream before each chunk of steps; no graph capture. */
+int gpe_batch_step_times(gpe_batch* batch, int64_t nsteps, const double* t_mid, int64_t stride);
 /* Asynchronous form: enqueue the steps on the batch's own stream and return;
  * gpe_batch_wait blocks until they are done.  Batches of different shapes
  * launched back to back overlap on the device.  Members must not be used
@@ -195,7 +202,8 @@ int gpe_sync(gpe_solve* solve);
 /* Measurement hooks (no reference counterpart; gpe/benchmark.py:59-69 times
  * whole batches).  gpe_step_timed runs gpe_step with HIP events around every
  * pass launch on the solve's stream and reports the summed milliseconds per
- * pass kind (9 kinds, see GPE_PASS_* in the sources) and launch counts. */
+ * pass kind (GPE_PASS_KINDS entries, in the order listed above) and launch
+ * counts. */
 int gpe_step_timed(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* ms_per_kind,
                    int64_t* launches_per_kind);
 /* Device-to-device streaming copy of `bytes` (multiple of 16), repeated

@@ -115,3 +115,59 @@ def test_server_results_same_alone_and_in_mix(require_gpu):
         assert m.steps == a.steps
         assert m.mu == pytest.approx(a.mu, rel=1e-12)
         assert m.energy == pytest.approx(a.energy, rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+def test_stirrers_of_different_speeds_match_solo_runs(precision, require_gpu):
+    """Members with their own time-dependent programs (beams at different
+    angular speeds, one member without a beam) batched: each is
+    bit-identical to the same run alone, for chunked multi-step calls."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    from gpe_simulator_amd.units import stirrer_expression
+    n, length, dt, steps = 64, 16.0, 5e-4, 300
+    x, k = orc.get_grids(n, length)
+    beams = [stirrer_expression(5.0, 3.0, w, 2.0) for w in (0.2, 0.6, 1.1)] + [None]
+    psi0 = orc.synthetic_state(n, length, seed=4, beta=400.0)
+
+    def member(beam, i):
+        s = make_solve(n, x, k, "real", dt * (1 + 0.1 * i), 400.0, static="x**2 + y**2 + z**2", dynamic=beam,
+                       precision=precision)
+        s.upload(psi0)
+        return s
+
+    batched = [member(b, i) for i, b in enumerate(beams)]
+    alone = [member(b, i) for i, b in enumerate(beams)]
+    times = np.stack([midpoint_times(0, 2 * steps, dt * (1 + 0.1 * i)) for i in range(len(beams))])
+    batch = _lib.Batch(batched)
+    with pytest.raises(AssertionError):
+        batch.step(10)                       # time-dependent members need their times
+    batch.step(steps, t_mid=times[:, :steps])
+    batch.step(steps, t_mid=np.ascontiguousarray(times[:, steps:]))
+    for i, (a, b) in enumerate(zip(batched, alone)):
+        b.step(steps, times[i, :steps])
+        b.step(steps, times[i, steps:])
+        np.testing.assert_array_equal(a.to_numpy(), b.to_numpy())
+    lz = [s.angular_momentum() for s in batched]
+    assert len({round(v, 12) for v in lz}) == len(lz)      # the speeds imprint different Lz
+    batch.close()
+
+
+def test_stirring_sweep_matches_sequential(require_gpu):
+    from gpe_simulator_amd.sweep import stir_batched, stir_sequential, stirrer_runs
+    runs = stirrer_runs([0.3, 0.7, 1.2], n=64, length=16.0, g=200.0, steps=200)
+    psi0 = orc.synthetic_state(64, 16.0, seed=6, beta=400.0)
+    batched = stir_batched(runs, psi0, precision="c64")
+    for run, got in zip(runs, batched):
+        alone = stir_sequential(run, psi0, precision="c64")
+        assert got == alone
+
+
+def test_sweep_over_mixed_sides(require_gpu):
+    """Requests of two grid sides relax in separate batches of one sweep,
+    each to the same result as alone."""
+    from gpe_simulator_amd.sweep import config_sweep, solve_sequential, sweep_batched
+    reqs = config_sweep(count=2, seed=5, n=64, length=20.0) + config_sweep(count=2, seed=6, n=96, length=20.0)
+    for req, res in zip(reqs, sweep_batched(reqs)):
+        assert res.converged
+        assert res.mu == pytest.approx(solve_sequential(req).mu, rel=1e-8)
