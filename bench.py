@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import re
 import sys
 import time
@@ -163,17 +164,37 @@ def stirring(count, n=128, steps=500):
             "Lz_range": [min(r["Lz"] for r in batched), max(r["Lz"] for r in batched)]}
 
 
-def cpu_baseline(psi0, samples):
-    """The FP64 oracle (numpy.fft, single thread) on a bounded sample."""
+def cpu_baseline(psi0, samples, workers=None):
+    """The FP64 oracle on a bounded sample: numpy.fft on one thread, or
+    scipy.fft with `workers` threads."""
     from oracle import gpe_oracle as orc
-    problem = orc.Problem(N, LENGTH, "real", DT, BETA, static=STATIC, dynamic=BEAM)
-    psi = psi0.copy()
-    psi = problem.step(psi, 0.0)                     # warm-up step
-    t0 = time.perf_counter()
-    for i in range(1, samples + 1):
-        psi = problem.step(psi, i * DT)
-    elapsed = time.perf_counter() - t0
+    orc.use_threads(workers)
+    try:
+        problem = orc.Problem(N, LENGTH, "real", DT, BETA, static=STATIC, dynamic=BEAM)
+        psi = psi0.copy()
+        psi = problem.step(psi, 0.0)                     # warm-up step
+        t0 = time.perf_counter()
+        for i in range(1, samples + 1):
+            psi = problem.step(psi, i * DT)
+        elapsed = time.perf_counter() - t0
+    finally:
+        orc.use_threads(None)
     return N ** 3 * samples / elapsed, elapsed
+
+
+def accuracy(psi0, x, run64, run128):
+    """complex64 against the complex128 engine after the same steps (the
+    complex128 path is held to the FP64 oracle by the parity tests)."""
+    dx3 = float(x[1] - x[0]) ** 3
+    n0 = float(np.sum(np.abs(psi0) ** 2) * dx3)
+    (psi64, d64), (psi128, d128) = run64, run128
+    rho64, rho128 = np.abs(psi64) ** 2, np.abs(psi128) ** 2
+    return {"against": "complex128 engine after the same steps",
+            "mu_rel": abs(d64["mu"] - d128["mu"]) / abs(d128["mu"]),
+            "E_rel": abs(d64["E"] - d128["E"]) / abs(d128["E"]),
+            "density_rel_l2": float(np.linalg.norm(rho64 - rho128) / np.linalg.norm(rho128)),
+            "norm_drift_c64": abs(d64["norm"] - n0) / n0,
+            "norm_drift_c128": abs(d128["norm"] - n0) / n0}
 
 
 def main(argv=None):
@@ -181,7 +202,7 @@ def main(argv=None):
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=2000)
     parser.add_argument("--warmup", type=int, default=100)
-    parser.add_argument("--c128-steps", type=int, default=1000)
+    parser.add_argument("--c128-steps", type=int, default=None, help="default: --steps")
     parser.add_argument("--timed-steps", type=int, default=50)
     parser.add_argument("--cpu-samples", type=int, default=3)
     parser.add_argument("--skip-cpu", action="store_true")
@@ -203,6 +224,7 @@ def main(argv=None):
 
     log(f"c64 run: {args.warmup} warm-up + {args.steps} timed steps")
     solve64, el64, diag64 = run_engine("c64", psi0, x, k, args.steps, args.warmup)
+    final64 = (solve64.to_numpy(), diag64)
     value = N ** 3 * args.steps / el64
     log(f"c64: {el64 * 1e3 / args.steps:.4f} ms/step, {value:.4e} grid-pt-steps/s, mu={diag64['mu']:.9g}")
 
@@ -220,22 +242,30 @@ def main(argv=None):
     ms_copy = _lib.default_context().bench_copy(copy_bytes, reps=20)
     copy_gbs = 2 * copy_bytes / (ms_copy * 1e-3) / 1e9
 
-    result_c128 = None
+    result_c128, acc = None, None
+    c128_steps = args.c128_steps or args.steps
     if not args.skip_c128:
-        log(f"c128 run: {args.c128_steps} steps")
-        solve128, el128, diag128 = run_engine("c128", psi0, x, k, args.c128_steps, args.warmup)
-        v128 = N ** 3 * args.c128_steps / el128
-        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / args.c128_steps,
+        log(f"c128 run: {c128_steps} steps")
+        solve128, el128, diag128 = run_engine("c128", psi0, x, k, c128_steps, args.warmup)
+        v128 = N ** 3 * c128_steps / el128
+        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / c128_steps,
                        "speedup_c64_over_c128": value / v128, "mu": diag128["mu"]}
+        if c128_steps == args.steps:
+            acc = accuracy(psi0, x, final64, (solve128.to_numpy(), diag128))
         solve128.close()
+    del final64
 
-    cpu = None
+    cpu, cpu_threads = None, None
     if not args.skip_cpu:
         log(f"CPU oracle baseline: {args.cpu_samples} steps at {N}^3")
         rate, el = cpu_baseline(psi0, args.cpu_samples)
         cpu = {"value": rate, "unit": "grid-pt-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_samples} Strang steps of the same 256^3 stirring workload "
                          f"(complex128 numpy.fft, 1 thread) after 1 warm-up step, {el:.1f} s"}
+        workers = min(16, os.cpu_count() or 1)
+        rate, el = cpu_baseline(psi0, args.cpu_samples, workers=workers)
+        cpu_threads = {"value": rate, "unit": "grid-pt-steps/s", "cores": workers, "kind": "port",
+                       "sample": f"the same {args.cpu_samples} steps with scipy.fft on {workers} threads, {el:.1f} s"}
 
     served = None
     if args.serve_requests > 0:
@@ -277,6 +307,8 @@ def main(argv=None):
         "mu_c64": diag64["mu"],
         "c128": result_c128,
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_threads,
+        "accuracy_c64": acc,
         "p50_solve_ms": served["p50_ms"] if served else None,
         "p99_solve_ms": served["p99_ms"] if served else None,
         "serving": served,

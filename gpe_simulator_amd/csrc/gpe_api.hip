@@ -233,7 +233,6 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
     d.vdyn_ss = diag ? 0 : s->n3;
   }
   d.red = s->red;
-  d.counter = s->counter;
   d.scal = s->scal;
   d.hist = s->hist;
   d.nkocc = s->nkocc;

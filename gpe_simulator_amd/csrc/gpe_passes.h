@@ -44,7 +44,7 @@ enum PassFlags : int {
   F_PRESCALE = 1,  // multiply the loaded field by scal[1] (pending 1/sqrt(norm))
   F_IN_WORK = 2,   // read from the workspace instead of psi
   F_OUT_WORK = 4,  // write to the workspace instead of psi
-  F_NORM = 8,      // Z_LAST: FP64 norm reduction, last block computes scale
+  F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -66,7 +66,6 @@ struct SolveDev {
   const R* vdyn;           // time-dependent volume [slot][n^3] (nullable)
   long long pxy_ss, pz_ss, vdyn_ss;   // slot strides (0 = static table)
   double* red;             // per-block partial sums
-  unsigned int* counter;   // last-block ticket (self-resetting)
   double* scal;            // [0] norm, [1] 1/sqrt(norm)
   double* hist;            // norm history per step (nullable)
   double* nkocc;           // [block][nbins] |F|^2 partials

@@ -104,14 +104,18 @@ class Switch:
     the complex64 field's own floor (a FieldFloor, once measured) if that is
     higher: from there on complex64 steps no longer bring the field closer.
     The complex128 polish then runs until its own, noise-free |dmu| < tol.
-    Without a usable fit the switch happens at once."""
+    Without a usable fit the switch happens at once.  A fit is trusted for
+    at most `horizon` times the steps it took to see the floor: a line that
+    predicts a longer complex64 phase was fitted to a transient (a slowly
+    relaxing member whose clean history is short), and the solve switches
+    at that bound instead."""
 
-    def __init__(self, tol, plateau=10, clean_factor=30.0, fit_points=6, field_floor=None):
+    def __init__(self, tol, plateau=10, clean_factor=30.0, fit_points=6, field_floor=None, horizon=1.5):
         self.tol, self.plateau, self.clean_factor, self.fit_points = tol, plateau, clean_factor, fit_points
-        self.field_floor = field_floor
+        self.field_floor, self.horizon = field_floor, horizon
         self.steps, self.dmu = [], []
         self.fit = None                  # (slope, intercept) of log|dmu| against steps
-        self.detected = False
+        self.detected = None             # step at which the floor was recognised
         self.late = False                # switched where the line reached tol
 
     def update(self, steps, dmu):
@@ -121,18 +125,18 @@ class Switch:
         if dmu < self.tol:
             self.late = True
             return True
-        if not self.detected:
+        if self.detected is None:
             p = self.plateau
             if len(self.dmu) <= p or min(self.dmu[-p:]) < min(self.dmu[:-p]):
                 return False
-            self.detected = True
+            self.detected = steps
             self.fit = self.predict()
             if self.fit is None:
                 return True
         floor = self.field_floor.value if self.field_floor is not None else None
         level = max(self.tol, floor) if floor is not None else self.tol
         self.late = floor is None
-        return steps >= self.step_at(level)
+        return steps >= min(self.step_at(level), self.horizon * self.detected)
 
     def step_at(self, level):
         slope, icpt = self.fit

@@ -88,3 +88,14 @@ def test_measured_field_floor_moves_the_switch_earlier():
 def test_mu_below_tol_marks_the_switch_late():
     switch = Switch(1e-9, field_floor=FieldFloor())
     assert switch.update(50, 1e-10) and switch.late
+
+
+def test_shallow_fit_is_bounded_by_the_horizon():
+    # a slow member: the clean part of its history is a transient that
+    # extrapolates far beyond where the floor was recognised
+    steps = np.arange(50, 40001, 50)
+    dmus = np.maximum(1e-3 * np.exp(-steps / 3000.0), 1e-6 * np.random.default_rng(2).uniform(0.2, 2.0, steps.size))
+    switch = Switch(2e-9, horizon=1.5)
+    at = feed(switch, steps, dmus)
+    assert switch.target > 1.5 * switch.detected
+    assert at == steps[np.searchsorted(steps, 1.5 * switch.detected)]
