@@ -32,7 +32,7 @@ EXPORTS = (
     "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
-    "gpe_solve_get_variants", "gpe_batch_step_times",
+    "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history",
 )
 
 
@@ -100,6 +100,7 @@ def _declare(lib):
         "gpe_solve_set_variants": (c_int, [vp, POINTER(i32), i32]),
         "gpe_solve_get_variants": (c_int, [vp, POINTER(i32), i32]),
         "gpe_batch_step_times": (c_int, [vp, i64, dp, i64]),
+        "gpe_batch_norm_history": (c_int, [vp, dp, i64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -399,6 +400,12 @@ class Batch:
 
     def wait(self):
         check(lib().gpe_batch_wait(self.handle))
+
+    def norm_history(self, count):
+        """Per-step norms of the last `count` steps, shape (members, count)."""
+        out = np.empty((len(self.solves), int(count)))
+        check(lib().gpe_batch_norm_history(self.handle, dptr(out), int(count)))
+        return out
 
     def step_timed(self, nsteps):
         ms = np.zeros(len(PASS_KINDS))

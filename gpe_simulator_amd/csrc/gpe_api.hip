@@ -1067,6 +1067,7 @@ struct gpe_batch {
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
   int graph_var[kPassKinds] = {0};      // pass variants the graph was captured with
+  std::vector<char> uploaded;           // descriptor bytes last copied to `desc`
   bool timing = false;
   hipEvent_t ev[2] = {nullptr, nullptr};
   double ms[kPassKinds] = {0};
@@ -1145,8 +1146,11 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
   }
   std::vector<char> host(dsz * b->solves.size());
   for (size_t i = 0; i < b->solves.size(); ++i) std::memcpy(host.data() + i * dsz, b->solves[i]->h_desc_step.data(), dsz);
-  HIPCHK(hipMemcpyAsync(b->desc, host.data(), host.size(), hipMemcpyHostToDevice, b->st));
-  HIPCHK(hipStreamSynchronize(b->st));
+  if (host != b->uploaded) {   // descriptors change only when a member reallocates
+    HIPCHK(hipMemcpyAsync(b->desc, host.data(), host.size(), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipStreamSynchronize(b->st));
+    b->uploaded.swap(host);
+  }
   return GPE_OK;
 }
 
@@ -1226,6 +1230,21 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
 int gpe_batch_wait(gpe_batch* b) {
   if (!b) return fail(GPE_ERR_INVALID, "null batch");
   HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return GPE_OK;
+}
+
+int gpe_batch_norm_history(gpe_batch* b, double* out, int64_t count) {
+  if (!b || (!out && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  for (gpe_solve* s : b->solves)
+    if (count > s->hist_len) return fail(GPE_ERR_STATE, "fewer steps recorded than requested");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (size_t i = 0; i < b->solves.size(); ++i) {
+    gpe_solve* s = b->solves[i];
+    HIPCHK(hipMemcpyAsync(out + i * count, s->hist + (s->hist_len - count), count * sizeof(double),
+                          hipMemcpyDeviceToHost, b->st));
+  }
   HIPCHK(hipStreamSynchronize(b->st));
   return GPE_OK;
 }

@@ -5,7 +5,6 @@ stage loop, plus --precision to pick the field precision (complex128, the
 reference's, or complex64 with FP64 reductions and time)."""
 
 import argparse
-import os
 from pathlib import Path
 
 

@@ -13,7 +13,7 @@ import csv
 
 import numpy as np
 
-from .field import DeviceField, asnumpy
+from .field import asnumpy
 from .helper import is_due
 from .nk_extraction import shell_summary
 from .state_io import save_wavefunction

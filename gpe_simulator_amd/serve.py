@@ -23,10 +23,9 @@ number of steps, so a result is the same alone and in the mix.
 
 from __future__ import annotations
 
-import math
 import time
 from collections import deque
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -155,11 +154,17 @@ class Server:
                 groups.setdefault((slot.req.n, slot.precision, slot.req.kind), []).append(slot)
             tick0 = time.perf_counter()
             # every batch is built (device allocation) before any group runs
-            launched = [self._batch_for(members) for members in groups.values()]
-            for batch in launched:
+            launched = [(members, self._batch_for(members)) for members in groups.values()]
+            for _, batch in launched:
                 batch.launch(self.check_every, graph=self.graph)
-            for batch in launched:
+            for _, batch in launched:
                 batch.wait()
+            # per-step norms of every imaginary-time member, one fetch per group
+            norms = {}
+            for members, batch in launched:
+                if members[0].req.kind == "imaginary":
+                    for slot, row in zip(members, batch.norm_history(self.check_every)):
+                        norms[id(slot)] = row
             if not realtime:
                 # without sleeping for arrivals, time still passes while the device works
                 virtual += time.perf_counter() - tick0
@@ -171,7 +176,7 @@ class Server:
                 req = slot.req
                 if req.kind == "imaginary":
                     try:
-                        mu = mu_from_norms(slot.solve.norm_history(self.check_every), req.dt)
+                        mu = mu_from_norms(norms[id(slot)], req.dt)
                     except FloatingPointError as err:
                         raise FloatingPointError(f"request {req.id} (n={req.n}, {slot.precision}, "
                                                  f"{slot.steps} steps): {err}") from None

@@ -250,14 +250,16 @@ def sweep_batched(reqs, check_every=50, precision="c64", polish="c128", graph=Tr
         for key, _ in launched:
             batches[key][1].wait()
         still, promote = [], []
-        for (ph, _n), members in launched:
+        for key, members in launched:
+            ph = key[0]
             bulk = ph + 1 < len(phases)
-            for i in members:
+            norms = batches[key][1].norm_history(check_every)
+            for row, i in enumerate(members):
                 req, res = reqs[i], results[i]
                 res.steps += check_every
                 if ph == 0 and bulk:
                     res.steps_c64 += check_every
-                mu = mu_from_norms(solves[i].norm_history(check_every), req.dt)
+                mu = mu_from_norms(norms[row], req.dt)
                 res.mu_estimate, res.dmu = float(mu[-1]), float(abs(mu[-1] - mu[-2]))
                 res.history.append(res.dmu)
                 if res.steps >= req.max_steps:

@@ -194,6 +194,11 @@ int gpe_batch_step_times(gpe_batch* batch, int64_t nsteps, const double* t_mid, 
  * until the wait returns. */
 int gpe_batch_launch(gpe_batch* batch, int64_t nsteps, int32_t use_graph);
 int gpe_batch_wait(gpe_batch* batch);
+/* Per-step norms of the last `count` steps of every member (imaginary time),
+ * member-major: out[i*count + j]; one synchronisation for the whole batch.
+ * Reference: the per-step normalise of make_imag_propagator
+ * (gpe/make_propagator.py:148-150), kept per member. */
+int gpe_batch_norm_history(gpe_batch* batch, double* out, int64_t count);
 int gpe_batch_step_timed(gpe_batch* batch, int64_t nsteps, double* ms_per_kind, int64_t* launches_per_kind);
 
 /* Wait for all work of the solve. */

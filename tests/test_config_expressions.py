@@ -5,8 +5,6 @@ Mirrors the behaviour the reference's config_editor tests pin
 rendering golden strings of tests/test_workflow.py:25-32.
 """
 
-import ast
-import copy
 import math
 
 import numpy as np

@@ -40,10 +40,11 @@ def test_batch_members_match_solo_runs(kind, precision, graph, require_gpu):
         for s in alone:
             s.step(20)
         # eager work between replays: the fields are read back and compared
-        for a, b in zip(batched, alone):
+        for row, (a, b) in enumerate(zip(batched, alone)):
             np.testing.assert_array_equal(a.to_numpy(), b.to_numpy())
             if kind == "imaginary":
                 np.testing.assert_array_equal(a.norm_history(20), b.norm_history(20))
+                np.testing.assert_array_equal(batch.norm_history(20)[row], b.norm_history(20))
     batch.close()
 
 

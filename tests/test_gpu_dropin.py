@@ -4,8 +4,6 @@ package's stage / parse_yaml / launch_stage / main."""
 
 import csv
 import json
-import sys
-from pathlib import Path
 from unittest.mock import patch
 
 import numpy as np
