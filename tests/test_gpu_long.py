@@ -32,16 +32,16 @@ def engine(problem, precision, kind=None):
                       static=problem.static_source, dynamic=problem.dynamic_source, precision=precision)
 
 
-def test_imaginary_ground_state_1000_steps_c128(require_gpu):
-    """64^3 on [-10,10)^3, V = x^2+y^2+z^2, beta = 200, Gaussian start,
-    dtau = 5e-4, renormalised every step (the CPU-runnable configuration in
-    reference units), 1000 steps."""
+def test_imaginary_ground_state_2000_steps_c128(require_gpu):
+    """The CPU-runnable configuration in reference units: 64^3 on [-10,10)^3,
+    V = x^2+y^2+z^2, beta = 200, Gaussian start, dtau = 5e-4, renormalised
+    every step, 2000 steps."""
     p = orc.Problem(64, 20.0, "imaginary", 5e-4, 200.0, static="x**2 + y**2 + z**2")
     psi0 = orc.gaussian(p.x, 1.0)
-    expected = p.run(psi0, 1000)
+    expected = p.run(psi0, 2000)
     solve = engine(p, "c128")
     solve.gaussian(1.0)
-    solve.step(1000)
+    solve.step(2000)
     got = solve.to_numpy()
     assert rel_l2(got, expected) < 1e-10
     e_ref, e_dev = p.energies(expected), solve.diagnostics(0.0, spectrum=False)
@@ -125,3 +125,55 @@ def test_time_reversibility_256_c128(require_gpu):
     back.upload(np.conj(mid))
     back.step(1, np.array([0.3 + 1.25e-4]))
     np.testing.assert_allclose(np.conj(back.to_numpy()), psi0, rtol=0, atol=1e-12 * np.abs(psi0).max())
+
+
+def test_quench_128_c64_against_c128(require_gpu):
+    """The quench configuration in reference units: 128^3 on [-12,12)^3,
+    V = x^2 + y^2 + 16 z^2 (omega = (1,1,4)), beta = 1000 (g = 500) relaxed in
+    complex128 to |dmu| < 2e-10, then beta raised by 10% and 10000 real-time
+    steps of dt = 5e-5 in complex128 and in complex64 from the same state.
+
+    complex128 conserves the norm to 1e-11 and the energy to 1e-7 relative
+    (Strang splitting is not exactly energy-conserving). complex64 stays
+    within 1e-6 of it in mu and in the energy per particle E/N, and within
+    1e-4 in the relative L2 distance of the density. Its norm drifts
+    (measured -1.2e-6 over these 10000 steps), so the bar on N is 5e-6. A
+    Fourier mode turns by k^2 dt per step, which is 1e-6..1e-4 rad for the
+    modes that hold most of the norm. A rotation that small moves the larger
+    float component by less than half an ulp, so round-to-nearest keeps that
+    component while the other moves. The value then follows the tangent
+    instead of the circle, and the rounding error has a systematic sign.
+    """
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    from gpe_simulator_amd.sweep import mu_from_norms
+    n, length, static = 128, 24.0, "x**2 + y**2 + 16.0*z**2"
+    x, k = orc.get_grids(n, length)
+    ground = make_solve(n, x, k, "imaginary", 5e-4, 1000.0, static=static, precision="c128")
+    ground.gaussian(1.0)
+    for chunk in range(400):
+        ground.step(50)
+        mu = mu_from_norms(ground.norm_history(50), 5e-4)
+        if abs(mu[-1] - mu[-2]) < 2e-10:
+            break
+    else:
+        pytest.fail("ground state did not converge")
+    psi0 = ground.to_numpy()
+    ground.close()
+    steps, dt = 10000, 5e-5
+    tm = midpoint_times(0, steps, dt)
+    out = {}
+    for precision in ("c128", "c64"):
+        s = make_solve(n, x, k, "real", dt, 1100.0, static=static, precision=precision)
+        s.upload(psi0)
+        d0 = s.diagnostics(0.0, spectrum=False)
+        s.step(steps, tm)
+        out[precision] = (d0, s.diagnostics(steps * dt, spectrum=False), s.to_numpy())
+        s.close()
+    (a0, a1, psi128), (b0, b1, psi64) = out["c128"], out["c64"]
+    assert a1["norm"] == pytest.approx(a0["norm"], rel=1e-11)
+    assert a1["E"] == pytest.approx(a0["E"], rel=1e-7)
+    assert b1["norm"] == pytest.approx(a1["norm"], rel=5e-6)
+    assert b1["E"] / b1["norm"] == pytest.approx(a1["E"] / a1["norm"], rel=1e-6)
+    assert b1["mu"] == pytest.approx(a1["mu"], rel=1e-6)
+    rho128, rho64 = np.abs(psi128) ** 2, np.abs(psi64) ** 2
+    assert np.linalg.norm(rho64 - rho128) / np.linalg.norm(rho128) < 1e-4
