@@ -43,9 +43,9 @@ __host__ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
   }
 }
 
-// a * w with the product formed in FP64 and rounded once to R.  Constant
-// factors (twiddles, kinetic phases) stay in FP64, so their rounding does not
-// repeat identically at every step of a complex64 run.
+// a * w with the product formed in FP64 and rounded once to R: for factors
+// that are themselves computed per point in FP64 (the kinetic phase) and for
+// the generic transform's FP64 twiddle table.
 template <typename R>
 __device__ __forceinline__ cpx<R> cmul_d(cpx<R> a, cpx<double> w) {
   const double x = a.x, y = a.y;
@@ -54,6 +54,72 @@ __device__ __forceinline__ cpx<R> cmul_d(cpx<R> a, cpx<double> w) {
 template <typename R>
 __device__ __forceinline__ cpx<R> cscale_d(cpx<R> a, double s) {
   return cpx<R>{R(double(a.x) * s), R(double(a.y) * s)};
+}
+
+// A constant complex factor (c, s) = c + i s in the field's arithmetic.
+// complex128: FP64.  complex64: each component as a float pair hi + lo, lo
+// the rounding error of hi, so the constant is carried to ~2^-48 and the
+// product needs float instructions only (one multiply and three FMAs per
+// component, against two conversions in, two FP64 operations and one
+// conversion out).  The product's rounding depends on the data, so, as with
+// FP64 constants, no rounding of the constant repeats identically every step.
+template <typename R>
+struct cconst {
+  R c, s;
+};
+template <>
+struct alignas(16) cconst<float> {
+  float ch, sh, cl, sl;
+};
+
+template <typename R>
+__host__ __device__ constexpr cconst<R> make_cconst(double c, double s) {
+  if constexpr (std::is_same_v<R, float>) {
+    const float ch = float(c), sh = float(s);
+    return cconst<float>{ch, sh, float(c - double(ch)), float(s - double(sh))};
+  } else {
+    return cconst<double>{c, s};
+  }
+}
+__device__ __forceinline__ cconst<float> conj_c(cconst<float> w) { return {w.ch, -w.sh, w.cl, -w.sl}; }
+__device__ __forceinline__ cconst<double> conj_c(cconst<double> w) { return {w.c, -w.s}; }
+
+__device__ __forceinline__ cpx<float> cmul_c(cpx<float> a, cconst<float> w) {
+  return cpx<float>{fmaf(a.x, w.ch, fmaf(-a.y, w.sh, fmaf(a.x, w.cl, -a.y * w.sl))),
+                    fmaf(a.x, w.sh, fmaf(a.y, w.ch, fmaf(a.x, w.sl, a.y * w.cl)))};
+}
+__device__ __forceinline__ cpx<double> cmul_c(cpx<double> a, cconst<double> w) {
+  return cpx<double>{a.x * w.c - a.y * w.s, a.x * w.s + a.y * w.c};
+}
+
+// A real constant in the same representation (float hi + lo for complex64).
+template <typename R>
+struct rconst {
+  R v;
+};
+template <>
+struct rconst<float> {
+  float hi, lo;
+};
+template <typename R>
+__host__ __device__ constexpr rconst<R> make_rconst(double h) {
+  if constexpr (std::is_same_v<R, float>) {
+    const float hi = float(h);
+    return rconst<float>{hi, float(h - double(hi))};
+  } else {
+    return rconst<double>{h};
+  }
+}
+__device__ __forceinline__ float scale_r(float x, rconst<float> h) { return fmaf(x, h.hi, x * h.lo); }
+__device__ __forceinline__ double scale_r(double x, rconst<double> h) { return x * h.v; }
+template <typename R>
+__device__ __forceinline__ cpx<R> scale_r(cpx<R> a, rconst<R> h) {
+  return cpx<R>{scale_r(a.x, h), scale_r(a.y, h)};
+}
+// a * h for a compile-time real constant h.
+template <typename R>
+__device__ __forceinline__ R scale_c(R x, double h) {
+  return scale_r(x, make_rconst<R>(h));
 }
 
 // ------------------------------------------------------------ static loop --
@@ -131,24 +197,25 @@ __device__ __forceinline__ void twiddle(cpx<R>& a) {
   } else if constexpr (4 * rr == 3 * m) {
     a = mul_i<-DIR>(a);
   } else if constexpr ((8 * rr) % m == 0) {
-    // odd multiples of pi/4: (c + DIR*i*s) with |c| = |s| = sqrt(1/2), FP64
+    // odd multiples of pi/4: (c + DIR*i*s) with |c| = |s| = sqrt(1/2)
     constexpr double c = ce_trig2pi(rr, m, false);
     constexpr double s = DIR * ce_trig2pi(rr, m, true);
     constexpr double h = 0.70710678118654752440084436210484904;
-    const double p = a.x, q = a.y;
+    const R p = a.x, q = a.y;
     if constexpr (c > 0 && s > 0) {
-      a = cpx<R>{R((p - q) * h), R((p + q) * h)};
+      a = cpx<R>{scale_c(p - q, h), scale_c(p + q, h)};
     } else if constexpr (c > 0 && s < 0) {
-      a = cpx<R>{R((p + q) * h), R((q - p) * h)};
+      a = cpx<R>{scale_c(p + q, h), scale_c(q - p, h)};
     } else if constexpr (c < 0 && s > 0) {
-      a = cpx<R>{R(-(p + q) * h), R((p - q) * h)};
+      a = cpx<R>{-scale_c(p + q, h), scale_c(p - q, h)};
     } else {
-      a = cpx<R>{R((q - p) * h), R(-(p + q) * h)};
+      a = cpx<R>{scale_c(q - p, h), -scale_c(p + q, h)};
     }
   } else {
     constexpr double c = ce_trig2pi(rr, m, false);
     constexpr double s = DIR * ce_trig2pi(rr, m, true);
-    a = cmul_d(a, cpx<double>{c, s});
+    constexpr cconst<R> w = make_cconst<R>(c, s);
+    a = cmul_c(a, w);
   }
 }
 
@@ -187,7 +254,7 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
     constexpr double h = 0.86602540378443864676372317075293618;  // sqrt(3)/2
     const cpx<R> a0 = v[0], t = v[1] + v[2], d = v[1] - v[2];
     const cpx<R> m = cpx<R>{a0.x - R(0.5) * t.x, a0.y - R(0.5) * t.y};
-    const cpx<R> r = mul_i<DIR>(cscale_d(d, h));
+    const cpx<R> r = mul_i<DIR>(cpx<R>{scale_c(d.x, h), scale_c(d.y, h)});
     v[0] = a0 + t;
     v[1] = m + r;
     v[2] = m - r;

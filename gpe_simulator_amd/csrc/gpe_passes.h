@@ -110,23 +110,28 @@ struct FastTF {
   __device__ static __forceinline__ bool valid(int, int, int) { return true; }
 
   // Twiddles W_N^{t*k1}, k1 >= 1 (k1 = 0 is the unit factor), staged once
-  // per block in LDS as twl[(k1-1)*TP + t].
+  // per block in LDS as twl[(k1-1)*TP + t], converted from the FP64 table to
+  // the field's constant representation (cconst).
+  using TwT = cconst<R>;
   static constexpr int TWL_ELEMS = (M - 1) * TP;
-  __device__ static __forceinline__ void stage_twiddles(cpx<double>* twl, const cpx<double>* tw) {
-    for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) twl[e] = gload(tw + (e % TP) * (e / TP + 1));
+  __device__ static __forceinline__ void stage_twiddles(TwT* twl, const cpx<double>* tw) {
+    for (int e = threadIdx.x; e < TWL_ELEMS; e += THREADS) {
+      const cpx<double> w = gload(tw + (e % TP) * (e / TP + 1));
+      twl[e] = make_cconst<R>(w.x, w.y);
+    }
     __syncthreads();
   }
 
   template <int DIR>
-  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* twl, int) {
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const TwT* twl, int) {
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
-      cpx<double> w = twl[(k1 - 1) * TP + t];
-      if constexpr (DIR > 0) w.y = -w.y;
-      v[k1] = cmul_d(v[k1], w);
-      // scheduling fence every 4 twiddles: the FP64 twiddles (4 registers
-      // each) are read from LDS as they are used, not all up front
+      TwT w = twl[(k1 - 1) * TP + t];
+      if constexpr (DIR > 0) w = conj_c(w);
+      v[k1] = cmul_c(v[k1], w);
+      // scheduling fence every 4 twiddles: the twiddles (4 registers each)
+      // are read from LDS as they are used, not all up front
       if constexpr (k1 % 4 == 0) __builtin_amdgcn_sched_barrier(0);
     });
     __syncthreads();
@@ -160,8 +165,9 @@ struct GenTF {
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + 64 * m; }
   __device__ static __forceinline__ bool valid(int t, int m, int n) { return t + 64 * m < n; }
+  using TwT = cpx<double>;
   static constexpr int TWL_ELEMS = 1;
-  __device__ static __forceinline__ void stage_twiddles(cpx<double>*, const cpx<double>*) {}
+  __device__ static __forceinline__ void stage_twiddles(TwT*, const cpx<double>*) {}
 
   template <int DIR>
   __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* __restrict__ tw,
@@ -291,7 +297,7 @@ __device__ __forceinline__ void load_tile(cpx<R>* v, const cpx<R>* src, long lon
 template <class TF, int AXIS, int KIND, typename R>
 __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R> a) {
   __shared__ cpx<R> lds[TF::LDS_ELEMS];
-  __shared__ cpx<double> twl_lds[TF::TWL_ELEMS];
+  __shared__ typename TF::TwT twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
   constexpr int M = TF::M;
@@ -333,14 +339,17 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // live at once.
   constexpr int CHUNK = 4;
   TF::stage_twiddles(twl_lds, s.tw);
-  const cpx<double>* twp = TF::kFast ? static_cast<const cpx<double>*>(twl_lds) : s.tw;
+  const typename TF::TwT* twp;
+  if constexpr (TF::kFast) twp = twl_lds;
+  else twp = s.tw;
   const cpx<R>* src = (a.flags & F_IN_WORK) ? s.work : s.psi;
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
   const bool has_vim = s.vim != nullptr;
   const bool prescale = (a.flags & F_PRESCALE) != 0;
-  // the pending 1/sqrt(norm) is applied in FP64 so the renormalised state
-  // has unit norm to rounding, not to the float representation of the scale
-  const double sc = prescale ? s.scal[1] : 1.0;
+  // the pending 1/sqrt(norm) is applied from its FP64 value (float hi + lo
+  // for complex64) so the renormalised state has unit norm to rounding, not
+  // to the float representation of the scale
+  const rconst<R> sc = make_rconst<R>(prescale ? s.scal[1] : 1.0);
 
   // Reductions accumulated over all tiles of the block.
   double acc[3] = {0.0, 0.0, 0.0};
@@ -410,7 +419,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         TF::template fft<+1>(v, lp, t, twp, n);
       } else if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
+        for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
       const long long rowoff = g.base;
       const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
+        for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
       const long long rowoff = g.base;
       const double pxa = double(gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj)));
@@ -471,7 +480,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
       if (prescale) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) v[m] = cscale_d(v[m], sc);
+        for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
       TF::template fft<-1>(v, lp, t, twp, n);
       const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);
