@@ -94,6 +94,61 @@ int variants_for(int prec, int n, const PassVariant** first) {
   return count;
 }
 
+// Per-pass timing: every timed launch gets an event pair that its own
+// dispatch timestamps at start and end (hipExtLaunchKernel), read after the
+// last launch.  Separate event records around a launch add their own packet
+// processing (several microseconds per launch); the dispatch timestamps do
+// not, and nothing synchronises between launches, so each pass is timed as
+// it runs back to back with its neighbours.  Events come from a pool that
+// grows on use.
+struct PassTimer {
+  std::vector<hipEvent_t> pool;
+  std::vector<int> kinds;          // pass kind of each recorded pair
+  size_t used = 0;
+  double ms[kPassKinds] = {0};
+  long long cnt[kPassKinds] = {0};
+
+  void reset() {
+    kinds.clear();
+    used = 0;
+    for (int k = 0; k < kPassKinds; ++k) {
+      ms[k] = 0;
+      cnt[k] = 0;
+    }
+  }
+  // The next pair of the pool for a launch of `kind`.
+  hipError_t pair(int kind, hipEvent_t* t0, hipEvent_t* t1) {
+    while (pool.size() < used + 2) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      pool.push_back(e);
+    }
+    *t0 = pool[used];
+    *t1 = pool[used + 1];
+    used += 2;
+    kinds.push_back(kind);
+    return hipSuccess;
+  }
+  // Waits for the stream and adds every recorded pair to ms/cnt.
+  hipError_t collect(hipStream_t st) {
+    hipError_t r = hipStreamSynchronize(st);
+    for (size_t i = 0; r == hipSuccess && i < kinds.size(); ++i) {
+      float t = 0.f;
+      r = hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]);
+      ms[kinds[i]] += t;
+      cnt[kinds[i]] += 1;
+    }
+    kinds.clear();
+    used = 0;
+    return r;
+  }
+  void destroy() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+
 }  // namespace
 
 struct gpe_ctx {
@@ -113,8 +168,9 @@ struct gpe_solve {
   int nvt = 0;
   int var[kPassKinds] = {0};              // variant used by each pass kind
   long long blocks(int kind, int n) const { return vt[var[kind]].blocks(kind, n); }
-  hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream) const {
-    return vt[var[kind]].launch(kind, args, n, batch, stream);
+  hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream, hipEvent_t t0 = nullptr,
+                    hipEvent_t t1 = nullptr) const {
+    return vt[var[kind]].launch(kind, args, n, batch, stream, t0, t1);
   }
 
   void* psi = nullptr;
@@ -160,9 +216,7 @@ struct gpe_solve {
 
   bool pending = false;                   // psi_true = psi * scal[1]
   bool timing = false;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  double ms[kPassKinds] = {0};
-  long long cnt[kPassKinds] = {0};
+  PassTimer timer;
 };
 
 namespace {
@@ -274,17 +328,10 @@ int ensure_red(gpe_solve* s, long long need) {
 
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index};
-  if (s->timing) HIPCHK(hipEventRecord(s->ev[0], s->st));
-  hipError_t e = s->launch(kind, &a, s->n, 1, s->st);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (s->timing) HIPCHK(s->timer.pair(kind, &t0, &t1));
+  hipError_t e = s->launch(kind, &a, s->n, 1, s->st, t0, t1);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
-  if (s->timing) {
-    HIPCHK(hipEventRecord(s->ev[1], s->st));
-    HIPCHK(hipEventSynchronize(s->ev[1]));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
-    s->ms[kind] += ms;
-    s->cnt[kind] += 1;
-  }
   return GPE_OK;
 }
 
@@ -748,8 +795,7 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
       (r = dmalloc(&s->desc_diag, sizeof(SolveDev<double>))))
     return bail(r);
   if (hipMemcpy(s->xa, d->x_array, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->ka, d->k_array, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipEventCreate(&s->ev[0]) != hipSuccess || hipEventCreate(&s->ev[1]) != hipSuccess)
+      hipMemcpy(s->ka, d->k_array, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return bail(fail(GPE_ERR_HIP, "setup copy failed"));
   const double inv_n3 = 1.0 / double(s->n3);
   const int imag = s->kind == GPE_IMAG ? 1 : 0;
@@ -780,8 +826,7 @@ int gpe_solve_destroy(gpe_solve* s) {
     if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
     if (s->tlist_ev[f]) (void)hipEventDestroy(s->tlist_ev[f]);
   }
-  for (auto ev : s->ev)
-    if (ev) (void)hipEventDestroy(ev);
+  s->timer.destroy();
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
   return GPE_OK;
@@ -959,18 +1004,19 @@ int gpe_step(gpe_solve* s, int64_t nsteps, const double* t_mid) {
 int gpe_step_timed(gpe_solve* s, int64_t nsteps, const double* t_mid, double* ms_per_kind,
                    int64_t* launches_per_kind) {
   if (!s || !ms_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
-  for (int k = 0; k < kPassKinds; ++k) {
-    s->ms[k] = 0;
-    s->cnt[k] = 0;
-  }
+  CHK(set_device(s));
+  s->timer.reset();
   s->timing = true;
-  int r = do_step(s, nsteps, t_mid);
+  const int r = do_step(s, nsteps, t_mid);
   s->timing = false;
+  const hipError_t e = s->timer.collect(s->st);
+  CHK(r);
+  if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass timing: ") + hipGetErrorString(e));
   for (int k = 0; k < kPassKinds; ++k) {
-    ms_per_kind[k] = s->ms[k];
-    launches_per_kind[k] = s->cnt[k];
+    ms_per_kind[k] = s->timer.ms[k];
+    launches_per_kind[k] = s->timer.cnt[k];
   }
-  return r;
+  return GPE_OK;
 }
 
 int gpe_norm_history(gpe_solve* s, double* out, int64_t count) {
@@ -1069,9 +1115,7 @@ struct gpe_batch {
   int graph_var[kPassKinds] = {0};      // pass variants the graph was captured with
   std::vector<char> uploaded;           // descriptor bytes last copied to `desc`
   bool timing = false;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  double ms[kPassKinds] = {0};
-  long long cnt[kPassKinds] = {0};
+  PassTimer timer;
 };
 
 int gpe_batch_destroy(gpe_batch* b) {
@@ -1081,8 +1125,7 @@ int gpe_batch_destroy(gpe_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   dfree(b->desc);
-  for (auto e : b->ev)
-    if (e) (void)hipEventDestroy(e);
+  b->timer.destroy();
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
   return GPE_OK;
@@ -1109,9 +1152,8 @@ int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_
     gpe_batch_destroy(b);
     return code;
   };
-  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
-    return bail(fail(GPE_ERR_HIP, "batch stream/event creation failed"));
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(GPE_ERR_HIP, "batch stream creation failed"));
   const size_t dsz = b->prec == GPE_C64 ? sizeof(SolveDev<float>) : sizeof(SolveDev<double>);
   if (int r = dmalloc(&b->desc, dsz * count)) return bail(r);
   *out = b;
@@ -1156,17 +1198,10 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   PassArgsRaw a{b->desc, sa, sb, flags, hist};
-  if (b->timing) HIPCHK(hipEventRecord(b->ev[0], b->st));
-  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (b->timing) HIPCHK(b->timer.pair(kind, &t0, &t1));
+  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, t0, t1);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
-  if (b->timing) {
-    HIPCHK(hipEventRecord(b->ev[1], b->st));
-    HIPCHK(hipEventSynchronize(b->ev[1]));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-    b->ms[kind] += ms;
-    b->cnt[kind] += 1;
-  }
   return GPE_OK;
 }
 
@@ -1300,18 +1335,19 @@ int gpe_batch_step_times(gpe_batch* b, int64_t nsteps, const double* t_mid, int6
 
 int gpe_batch_step_timed(gpe_batch* b, int64_t nsteps, double* ms_per_kind, int64_t* launches_per_kind) {
   if (!b || !ms_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
-  for (int k = 0; k < kPassKinds; ++k) {
-    b->ms[k] = 0;
-    b->cnt[k] = 0;
-  }
+  HIPCHK(hipSetDevice(b->ctx->device));
+  b->timer.reset();
   b->timing = true;
   const int r = gpe_batch_step(b, nsteps, 0);
   b->timing = false;
+  const hipError_t e = b->timer.collect(b->st);
+  CHK(r);
+  if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass timing: ") + hipGetErrorString(e));
   for (int k = 0; k < kPassKinds; ++k) {
-    ms_per_kind[k] = b->ms[k];
-    launches_per_kind[k] = b->cnt[k];
+    ms_per_kind[k] = b->timer.ms[k];
+    launches_per_kind[k] = b->timer.cnt[k];
   }
-  return r;
+  return GPE_OK;
 }
 
 int gpe_sync(gpe_solve* s) {

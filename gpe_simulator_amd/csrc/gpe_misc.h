@@ -38,11 +38,15 @@ void launch_copy(const void* in, void* out, long long bytes, hipStream_t st);
 
 // Pass launchers: one per (precision, side, variant) instantiation; side 0 is
 // the generic transform (one variant).
-using PassLauncher = hipError_t (*)(int kind, const void* args, int n, int batch, hipStream_t st);
+// t0 / t1 (nullable): events the launch's own dispatch timestamps at its start
+// and end (hipExtLaunchKernel), for per-pass timing.
+using PassLauncher = hipError_t (*)(int kind, const void* args, int n, int batch, hipStream_t st, hipEvent_t t0,
+                                    hipEvent_t t1);
 using PassBlocks = long long (*)(int kind, int n);
 
 #define GPE_DECL_PASS(P, N, V, ...)                                                                 \
-  hipError_t launch_pass_p##P##_n##N##_v##V(int kind, const void* args, int n, int batch, hipStream_t st); \
+  hipError_t launch_pass_p##P##_n##N##_v##V(int kind, const void* args, int n, int batch, hipStream_t st,     \
+                                            hipEvent_t t0, hipEvent_t t1);                                 \
   long long pass_blocks_p##P##_n##N##_v##V(int kind, int n);
 GPE_DECL_PASS(0, 0, 0)
 GPE_DECL_PASS(1, 0, 0)

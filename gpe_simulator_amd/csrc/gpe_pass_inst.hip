@@ -3,6 +3,8 @@
 // (0 = generic direct-DFT transform, otherwise a fast side) and GPE_V (tile
 // variant from the list in gpe_shapes.h), so the heavy template
 // instantiations compile in parallel.
+#include <hip/hip_ext.h>
+
 #include "gpe_passes.h"
 #include "gpe_shapes.h"
 
@@ -36,9 +38,12 @@ using TF = typename Shape<GPE_PREC, GPE_N, GPE_V>::TF;
 #define GPE_LAUNCHER GPE_CAT(launch_pass_, GPE_SUFFIX)
 
 template <int AXIS, int KIND>
-static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a) {
+static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEvent_t t0, hipEvent_t t1) {
   const dim3 grid((unsigned)tile_count<TF, AXIS>(n), 1, (unsigned)batch);
-  hipLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, a);
+  if (t0 || t1)
+    hipExtLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, t0, t1, 0, a);
+  else
+    hipLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, a);
 }
 
 static int axis_of(int kind) {
@@ -50,26 +55,36 @@ static int axis_of(int kind) {
 }
 
 // Launches one pass over `batch` solves of side n sharing one descriptor
-// array; one block per tile of B pencils.
-hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_t st) {
+// array; one block per tile of B pencils.  With t0/t1 the dispatch itself
+// timestamps its start and end into the events (the norm finisher, when it
+// follows Z_LAST, carries the end event).
+hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_t st, hipEvent_t t0,
+                        hipEvent_t t1) {
   const PassArgs<Real>& a = *static_cast<const PassArgs<Real>*>(args);
   switch (kind) {
-    case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a); break;
-    case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a); break;
+    case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a, t0, t1); break;
+    case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a, t0, t1); break;
     case K_Z_LAST:
-      go<2, K_Z_LAST>(n, batch, st, a);
-      if (a.flags & F_NORM)
-        hipLaunchKernelGGL((k_norm_finish<Real>), dim3((unsigned)batch), dim3(256), 0, st, a,
-                           (int)tile_count<TF, 2>(n));
+      if (a.flags & F_NORM) {
+        go<2, K_Z_LAST>(n, batch, st, a, t0, nullptr);
+        const dim3 grid((unsigned)batch), block(256);
+        const int nblk = (int)tile_count<TF, 2>(n);
+        if (t1)
+          hipExtLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, nullptr, t1, 0, a, nblk);
+        else
+          hipLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, a, nblk);
+      } else {
+        go<2, K_Z_LAST>(n, batch, st, a, t0, t1);
+      }
       break;
-    case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a); break;
-    case K_Z_FWD: go<2, K_Z_FWD>(n, batch, st, a); break;
-    case K_Y_FWD: go<1, K_Y_FWD>(n, batch, st, a); break;
-    case K_Y_INV: go<1, K_Y_INV>(n, batch, st, a); break;
-    case K_X_KIN: go<0, K_X_KIN>(n, batch, st, a); break;
-    case K_X_DIAG: go<0, K_X_DIAG>(n, batch, st, a); break;
-    case K_Y_LZ: go<1, K_Y_LZ>(n, batch, st, a); break;
-    case K_X_LZ: go<0, K_X_LZ>(n, batch, st, a); break;
+    case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a, t0, t1); break;
+    case K_Z_FWD: go<2, K_Z_FWD>(n, batch, st, a, t0, t1); break;
+    case K_Y_FWD: go<1, K_Y_FWD>(n, batch, st, a, t0, t1); break;
+    case K_Y_INV: go<1, K_Y_INV>(n, batch, st, a, t0, t1); break;
+    case K_X_KIN: go<0, K_X_KIN>(n, batch, st, a, t0, t1); break;
+    case K_X_DIAG: go<0, K_X_DIAG>(n, batch, st, a, t0, t1); break;
+    case K_Y_LZ: go<1, K_Y_LZ>(n, batch, st, a, t0, t1); break;
+    case K_X_LZ: go<0, K_X_LZ>(n, batch, st, a, t0, t1); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -205,10 +205,11 @@ int gpe_batch_step_timed(gpe_batch* batch, int64_t nsteps, double* ms_per_kind, 
 int gpe_sync(gpe_solve* solve);
 
 /* Measurement hooks (no reference counterpart; gpe/benchmark.py:59-69 times
- * whole batches).  gpe_step_timed runs gpe_step with HIP events around every
- * pass launch on the solve's stream and reports the summed milliseconds per
- * pass kind (GPE_PASS_KINDS entries, in the order listed above) and launch
- * counts. */
+ * whole batches).  gpe_step_timed runs gpe_step with a HIP event pair around
+ * every pass launch on the solve's stream, without synchronising between
+ * launches, and reports the summed milliseconds per pass kind
+ * (GPE_PASS_KINDS entries, in the order listed above) and launch counts.
+ * gpe_batch_step_timed does the same for a batch (eager launches, no graph). */
 int gpe_step_timed(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* ms_per_kind,
                    int64_t* launches_per_kind);
 /* Device-to-device streaming copy of `bytes` (multiple of 16), repeated
