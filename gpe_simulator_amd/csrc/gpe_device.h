@@ -328,13 +328,32 @@ __device__ __forceinline__ void gstore(cpx<R>* p, cpx<R> c) {
 }
 
 // ----------------------------------------------------------------- sincos --
-// sin and cos of the local-step phase.  Cody-Waite reduction by pi/2 with a
-// three-part split of pi/2 (FMA form), then Taylor polynomials on
-// |r| <= pi/4 whose truncation error is below half an ulp of the format.
-// Arguments beyond the reduction's exact range take the library path (kept
-// out of line so the common path stays small).
-__device__ __attribute__((noinline)) void wide_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __attribute__((noinline)) void wide_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+// sin and cos of the local-step phase, with no library call on any path (a
+// call in the per-point loop makes every value live across it spill to
+// scratch memory, even when the call never runs).
+//
+//   |x| < 0.78     Taylor polynomials directly (split-step phases are usually
+//                  this small);
+//   |x| < 2^40     Cody-Waite reduction by pi/2 in FP64, then the polynomials
+//                  and the quadrant fix-up;
+//   otherwise      NaN (also for non-finite x): a phase of 1e12 rad per half
+//                  step has no meaningful value in either precision.
+//
+// The reduction uses a three-part split of pi/2 in FMA form: each FMA forms
+// its product q * p_i exactly, so r = x - q*pi/2 carries an absolute error
+// near 1e-16 while q = rint(x * 2/pi) is exact.  Below 2^40 the rounding of
+// x * 2/pi moves q by at most one only when r is within 1.2e-4 of +-pi/4, so
+// |r| stays inside the polynomials' accurate range.  The quadrant is q mod 4,
+// taken in FP64 (q may exceed the int range).
+__device__ __forceinline__ double reduce_half_pi(double x, int* quad) {
+  const double q = rint(x * 0.63661977236758134308);
+  double r = fma(q, -1.5707963267948966, x);
+  r = fma(q, -6.123233995736766e-17, r);
+  r = fma(q, 1.4973849048591698e-33, r);
+  *quad = int(q - 4.0 * floor(q * 0.25));
+  return r;
+}
+constexpr double kPhaseLimit = 1099511627776.0;   // 2^40
 
 // Polynomials on |r| <= pi/4 (Taylor to r^9 / r^10 in float, r^19 / r^18 in
 // double).
@@ -351,31 +370,29 @@ __device__ __forceinline__ void sincos_poly(float r, float* s, float* c) {
   *c = fmaf(r2, pc, 1.0f);
 }
 
-// Cody-Waite reduction by pi/2 (three-part constant), then the polynomials.
-// For |x| < 0.78 the reduction leaves r = x exactly (q = rint(x*2/pi) = 0 and
-// fma(0, ., x) = x) and the quadrant fix-up is the identity, so the short
-// path returns the same bits without doing either; split-step phases are
-// usually that small.
+// Sign and order of (sin, cos) of r + quad * pi/2.
+template <typename R>
+__device__ __forceinline__ void quadrant(int quad, R sn, R cs, R* s, R* c) {
+  const R a = (quad & 1) ? cs : sn;
+  const R b = (quad & 1) ? sn : cs;
+  *s = (quad & 2) ? -a : a;
+  *c = ((quad + 1) & 2) ? -b : b;
+}
+
 __device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
   if (fabsf(x) < 0.78f) {
     sincos_poly(x, s, c);
     return;
   }
-  if (!(fabsf(x) < 8192.0f)) {
-    wide_sincos(x, s, c);
+  if (!(fabsf(x) < float(kPhaseLimit))) {
+    *s = *c = __builtin_nanf("");
     return;
   }
-  const float q = rintf(x * 0.636619772367581343f);
-  float r = fmaf(q, -1.5707963705062866f, x);
-  r = fmaf(q, 4.371138828673793e-08f, r);
-  r = fmaf(q, 1.7151245100058819e-15f, r);
+  int quad;
+  const float r = float(reduce_half_pi(double(x), &quad));
   float sn, cs;
   sincos_poly(r, &sn, &cs);
-  const int quad = int(q) & 3;
-  const float a = (quad & 1) ? cs : sn;
-  const float b = (quad & 1) ? sn : cs;
-  *s = (quad & 2) ? -a : a;
-  *c = ((quad + 1) & 2) ? -b : b;
+  quadrant(quad, sn, cs, s, c);
 }
 
 __device__ __forceinline__ void sincos_poly(double r, double* s, double* c) {
@@ -405,21 +422,15 @@ __device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
     sincos_poly(x, s, c);
     return;
   }
-  if (!(fabs(x) < 1.0e9)) {
-    wide_sincos(x, s, c);
+  if (!(fabs(x) < kPhaseLimit)) {
+    *s = *c = __builtin_nan("");
     return;
   }
-  const double q = rint(x * 0.63661977236758134308);
-  double r = fma(q, -1.5707963267948966, x);
-  r = fma(q, -6.123233995736766e-17, r);
-  r = fma(q, 1.4973849048591698e-33, r);
+  int quad;
+  const double r = reduce_half_pi(x, &quad);
   double sn, cs;
   sincos_poly(r, &sn, &cs);
-  const int quad = int(q) & 3;
-  const double a = (quad & 1) ? cs : sn;
-  const double b = (quad & 1) ? sn : cs;
-  *s = (quad & 2) ? -a : a;
-  *c = ((quad + 1) & 2) ? -b : b;
+  quadrant(quad, sn, cs, s, c);
 }
 
 // ------------------------------------------------------------- reductions --
