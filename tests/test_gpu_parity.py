@@ -135,17 +135,44 @@ def test_gaussian_initial_state(require_gpu):
     assert solve.norm() == pytest.approx(1.0, abs=1e-13)
 
 
+@pytest.mark.parametrize("n", [18, 64])
 @pytest.mark.parametrize("precision,static,tol", [
-    # half-step phases up to a few radians: Cody-Waite reduction path
+    # half-step phases up to a few radians: FP64 Cody-Waite reduction by pi/2
     ("c128", "40*(x*x + y*y + z*z)", 1e-11),
     ("c64", "40*(x*x + y*y + z*z)", 2e-5),
-    # phases of ~1e4 rad and beyond: the wide-argument fallback (complex128;
-    # a float phase that large carries ~1e-3 rad of rounding by itself)
+    # phases of ~1e4 rad (complex128; a float phase that large carries ~1e-3
+    # rad of rounding by itself)
     ("c128", "2e7 + 40*(x*x + y*y)", 1e-10),
+    # phases of ~1e10 rad: q = rint(x * 2/pi) far beyond the int range; the
+    # phase value itself carries ~2e-6 rad of FP64 rounding at this size
+    ("c128", "2e13 + 40*(x*x + y*y)", 1e-5),
 ])
-def test_large_phases(precision, static, tol, require_gpu):
-    n, length = 18, 12.0
+def test_large_phases(n, precision, static, tol, require_gpu):
+    length = 12.0
     problem, solve = make(n, length, "real", 1e-3, 50.0, static, precision=precision)
     psi0 = random_state(n, length, seed=3)
     got, expected = run_both(problem, solve, psi0, 3)
     assert rel_l2(got, expected) < tol
+
+
+def test_c64_phase_of_a_million_radians(require_gpu):
+    """complex64 phases of ~1e6 rad take the FP64 reduction: the field stays
+    finite and the rotation keeps the norm.  (The float phase itself carries
+    ~0.06 rad of rounding, so the values are not compared.)"""
+    problem, solve = make(18, 12.0, "real", 1e-3, 0.0, "2e9 + 40*(x*x + y*y)", precision="c64")
+    solve.upload(random_state(18, 12.0, seed=4))
+    n0 = solve.norm()
+    solve.step(3)
+    assert np.isfinite(solve.to_numpy()).all()
+    assert solve.norm() == pytest.approx(n0, rel=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+def test_phase_beyond_limit_gives_nan(precision, require_gpu):
+    """Half-step phases of 2^40 rad (1.1e12) and more have no meaningful
+    value in either precision: the field turns NaN instead of carrying a
+    silent number (here 2.5e12 rad)."""
+    problem, solve = make(18, 12.0, "real", 1e-3, 0.0, "5e15", precision=precision)
+    solve.upload(random_state(18, 12.0, seed=5))
+    solve.step(1)
+    assert np.isnan(solve.to_numpy()).all()
