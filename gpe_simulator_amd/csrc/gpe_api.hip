@@ -94,53 +94,68 @@ int variants_for(int prec, int n, const PassVariant** first) {
   return count;
 }
 
-// Per-pass timing: every timed launch gets an event pair that its own
-// dispatch timestamps at start and end (hipExtLaunchKernel), read after the
-// last launch.  Separate event records around a launch add their own packet
-// processing (several microseconds per launch); the dispatch timestamps do
-// not, and nothing synchronises between launches, so each pass is timed as
-// it runs back to back with its neighbours.  Events come from a pool that
-// grows on use.
+// Per-pass timing with the semantics of a kernel trace: each timed launch
+// hands its dispatch a stop event (hipExtLaunchKernel timestamps it at the
+// dispatch's end), and a pass lasts from the previous timed pass's stop to
+// its own, so back-to-back passes are timed as they run with nothing added
+// between them.  When other work was enqueued since the last timed pass (the
+// first pass of a call, a time-dependent table evaluation), a start marker is
+// recorded before the launch instead.  Events come from a pool that grows on
+// use and are read after the last launch; nothing synchronises in between.
 struct PassTimer {
+  struct Span {
+    int kind, start, stop;     // pool indices
+  };
   std::vector<hipEvent_t> pool;
-  std::vector<int> kinds;          // pass kind of each recorded pair
+  std::vector<Span> spans;
   size_t used = 0;
+  bool chained = false;        // the stream's last command was a timed pass
   double ms[kPassKinds] = {0};
   long long cnt[kPassKinds] = {0};
 
   void reset() {
-    kinds.clear();
+    spans.clear();
     used = 0;
+    chained = false;
     for (int k = 0; k < kPassKinds; ++k) {
       ms[k] = 0;
       cnt[k] = 0;
     }
   }
-  // The next pair of the pool for a launch of `kind`.
-  hipError_t pair(int kind, hipEvent_t* t0, hipEvent_t* t1) {
-    while (pool.size() < used + 2) {
+  hipError_t take(int* idx) {
+    if (used == pool.size()) {
       hipEvent_t e = nullptr;
       const hipError_t r = hipEventCreate(&e);
       if (r != hipSuccess) return r;
       pool.push_back(e);
     }
-    *t0 = pool[used];
-    *t1 = pool[used + 1];
-    used += 2;
-    kinds.push_back(kind);
+    *idx = (int)used++;
     return hipSuccess;
   }
-  // Waits for the stream and adds every recorded pair to ms/cnt.
+  // Before a timed launch of `kind`: the stop event to hand to the launch.
+  hipError_t begin(int kind, hipStream_t st, hipEvent_t* stop) {
+    int a = chained ? spans.back().stop : -1, b = -1;
+    hipError_t r = hipSuccess;
+    if (a < 0 && (r = take(&a)) == hipSuccess) r = hipEventRecord(pool[a], st);
+    if (r == hipSuccess) r = take(&b);
+    if (r != hipSuccess) return r;
+    spans.push_back({kind, a, b});
+    chained = true;
+    *stop = pool[b];
+    return hipSuccess;
+  }
+  // Waits for the stream and adds every span to ms/cnt.
   hipError_t collect(hipStream_t st) {
     hipError_t r = hipStreamSynchronize(st);
-    for (size_t i = 0; r == hipSuccess && i < kinds.size(); ++i) {
+    for (size_t i = 0; r == hipSuccess && i < spans.size(); ++i) {
       float t = 0.f;
-      r = hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]);
-      ms[kinds[i]] += t;
-      cnt[kinds[i]] += 1;
+      r = hipEventElapsedTime(&t, pool[spans[i].start], pool[spans[i].stop]);
+      ms[spans[i].kind] += t;
+      cnt[spans[i].kind] += 1;
     }
-    kinds.clear();
+    spans.clear();
     used = 0;
+    chained = false;
     return r;
   }
   void destroy() {
@@ -328,9 +343,9 @@ int ensure_red(gpe_solve* s, long long need) {
 
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index};
-  hipEvent_t t0 = nullptr, t1 = nullptr;
-  if (s->timing) HIPCHK(s->timer.pair(kind, &t0, &t1));
-  hipError_t e = s->launch(kind, &a, s->n, 1, s->st, t0, t1);
+  hipEvent_t stop = nullptr;
+  if (s->timing) HIPCHK(s->timer.begin(kind, s->st, &stop));
+  hipError_t e = s->launch(kind, &a, s->n, 1, s->st, nullptr, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
 }
@@ -464,6 +479,7 @@ bool any_dynamic(gpe_solve* s) { return s->dyn[0].active || s->dyn[1].active || 
 // a batch uses its stream so the tables stay ordered with its passes).
 int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream_t stream = nullptr) {
   hipStream_t st = stream ? stream : s->st;
+  s->timer.chained = false;    // the next timed pass does not start at a pass's end
   const int f = s->tlist_flip;
   s->tlist_flip ^= 1;
   HIPCHK(hipEventSynchronize(s->tlist_ev[f]));
@@ -1198,9 +1214,9 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   PassArgsRaw a{b->desc, sa, sb, flags, hist};
-  hipEvent_t t0 = nullptr, t1 = nullptr;
-  if (b->timing) HIPCHK(b->timer.pair(kind, &t0, &t1));
-  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, t0, t1);
+  hipEvent_t stop = nullptr;
+  if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &stop));
+  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, nullptr, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
 }
