@@ -1,6 +1,7 @@
 """Benchmark: 3D split-step stirring run at 256^3 on one MI355X.
 
-Workload (reference units, see DESIGN.md for the oscillator-unit mapping):
+Workload (reference units: beta = 2g, dt = dt'/2 and Omega = 2 Omega' from the
+oscillator-unit parameters g = 2000, dt' = 5e-4, Omega' = 0.5):
 256^3 grid on [-20, 20)^3, static trap V = x^2 + y^2 + z^2, beta = 4000,
 Gaussian stirring beam 10*exp(-((x-4cos t)^2 + (y-4sin t)^2)/2), dt = 2.5e-4,
 complex64 field with FP64 norm/energy and FP64 time, |psi(k)|^2 shell
