@@ -208,34 +208,36 @@ __device__ __forceinline__ float rexp(float x) {
 __device__ __forceinline__ double rexp(double x) { return exp(x); }
 
 // One local half-step at a point with total external potential `pot`
-// (static + dynamic, real part) and static imaginary part `vim`.
-template <typename R>
-__device__ __forceinline__ cpx<R> half_step(cpx<R> v, R pot, R vim, bool has_vim, const SolveDev<R>& s) {
+// (static + dynamic, real part) and static imaginary part `vim`.  IMAG and
+// VIM (an absorber is present) are compile-time choices: the pass picks the
+// instantiation once per tile, so no point evaluates a factor it discards.
+template <bool IMAG, bool VIM, typename R>
+__device__ __forceinline__ cpx<R> half_step(cpx<R> v, R pot, R vim, const SolveDev<R>& s) {
   const R a2 = v.x * v.x + v.y * v.y;
   const R e = (pot + s.beta * a2) * s.hdt;
-  if (s.imag) {
+  if constexpr (IMAG) {
     const R f = rexp(-e);
     return cpx<R>{v.x * f, v.y * f};
+  } else {
+    R sn, cs;
+    phase_sincos(e, &sn, &cs);
+    cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+    if constexpr (VIM) {
+      const R d = rexp(vim * s.hdt);
+      r = cpx<R>{r.x * d, r.y * d};
+    }
+    return r;
   }
-  R sn, cs;
-  phase_sincos(e, &sn, &cs);
-  cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
-  if (has_vim) {
-    const R d = rexp(vim * s.hdt);
-    r = cpx<R>{r.x * d, r.y * d};
-  }
-  return r;
 }
 
 // Two consecutive real-time half-steps (potentials pa then pb) as one phase:
 // the first multiplies |psi| by d = exp(vim dt/2), so the second sees
 // |psi|^2 d^2, and the product of the two factors is d^2 exp(-i(ea + eb)).
-template <typename R>
-__device__ __forceinline__ cpx<R> two_half_steps(cpx<R> v, R pa, R pb, R vim, bool has_vim,
-                                                 const SolveDev<R>& s) {
+template <bool VIM, typename R>
+__device__ __forceinline__ cpx<R> two_half_steps(cpx<R> v, R pa, R pb, R vim, const SolveDev<R>& s) {
   const R a2 = v.x * v.x + v.y * v.y;
   R d2 = R(1);
-  if (has_vim) {
+  if constexpr (VIM) {
     const R d = rexp(vim * s.hdt);
     d2 = d * d;
   }
@@ -243,7 +245,7 @@ __device__ __forceinline__ cpx<R> two_half_steps(cpx<R> v, R pa, R pb, R vim, bo
   R sn, cs;
   phase_sincos(e, &sn, &cs);
   cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
-  if (has_vim) r = cpx<R>{r.x * d2, r.y * d2};
+  if constexpr (VIM) r = cpx<R>{r.x * d2, r.y * d2};
   return r;
 }
 
@@ -432,32 +434,44 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       const R pxa = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
       R pxb = R(0);
       if constexpr (KIND == K_Z_MID) pxb = gload(s.pxy + (a.slot_b * s.pxy_ss + (long long)pi * n + pj));
+      auto points = [&](auto imag_tag, auto vim_tag) {
+        constexpr bool IMAG = decltype(imag_tag)::value;
+        constexpr bool VIM = decltype(vim_tag)::value;
 #pragma unroll
-      for (int m = 0; m < M; ++m) {
-        if (TF::valid(t, m, n)) {
-          const int k = TF::pos(t, m);
-          R vst = R(0), vi = R(0);
-          if (s.vre) vst = gload(s.vre + (rowoff + k));
-          if (has_vim) vi = gload(s.vim + (rowoff + k));
-          R pza;
-          if constexpr (kStagePz) pza = pz_lds[k];
-          else pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
-          R pa = pxa + pza + vst;
-          if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
-          cpx<R> w;
-          if constexpr (KIND == K_Z_MID) {
-            R pzb;
-            if constexpr (kStagePz) pzb = pz_lds[TF::N + k];
-            else pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
-            R pb = pxb + pzb + vst;
-            if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
-            w = two_half_steps(v[m], pa, pb, vi, has_vim, s);
-          } else {
-            w = half_step(v[m], pa, vi, has_vim, s);
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) {
+            const int k = TF::pos(t, m);
+            R vst = R(0), vi = R(0);
+            if (s.vre) vst = gload(s.vre + (rowoff + k));
+            if constexpr (VIM) vi = gload(s.vim + (rowoff + k));
+            R pza;
+            if constexpr (kStagePz) pza = pz_lds[k];
+            else pza = gload(s.pz + (a.slot_a * s.pz_ss + k));
+            R pa = pxa + pza + vst;
+            if (s.vdyn) pa += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
+            cpx<R> w;
+            if constexpr (KIND == K_Z_MID) {
+              R pzb;
+              if constexpr (kStagePz) pzb = pz_lds[TF::N + k];
+              else pzb = s.pz_ss == 0 ? pza : gload(s.pz + (a.slot_b * s.pz_ss + k));
+              R pb = pxb + pzb + vst;
+              if (s.vdyn) pb += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
+              w = two_half_steps<VIM>(v[m], pa, pb, vi, s);
+            } else {
+              w = half_step<IMAG, VIM>(v[m], pa, vi, s);
+            }
+            if constexpr (KIND == K_Z_LAST) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+            v[m] = w;
           }
-          if constexpr (KIND == K_Z_LAST) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
-          v[m] = w;
         }
+      };
+      // Z_MID is real-time only; an absorber exists only in real time.
+      if (KIND != K_Z_MID && s.imag) {
+        if constexpr (KIND != K_Z_MID) points(std::true_type{}, std::false_type{});
+      } else if (has_vim) {
+        points(std::false_type{}, std::true_type{});
+      } else {
+        points(std::false_type{}, std::false_type{});
       }
       if constexpr (KIND != K_Z_LAST) TF::template fft<-1>(v, lp, t, twp, n);
     } else if constexpr (KIND == K_Z_DIAG) {
