@@ -176,3 +176,33 @@ def test_phase_beyond_limit_gives_nan(precision, require_gpu):
     solve.upload(random_state(18, 12.0, seed=5))
     solve.step(1)
     assert np.isnan(solve.to_numpy()).all()
+
+
+@pytest.mark.parametrize("n", [64, 96])
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-10), ("c64", 3e-5)])
+def test_complex_static_volume_on_fast_sides(n, precision, tol, require_gpu):
+    """V_static = box - i*absorber as uploaded volumes (GPE_TGT_VOL and
+    GPE_TGT_VOL_IM) on a fast transform, with the trap as a program and the
+    moving beam: 6 real-time steps, so the merged two-half-step pass runs
+    with the absorber's decay (gpe/potentials.py:31-37 assembly)."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import midpoint_times
+    length, dt, beta = 16.0, 1e-3, 100.0
+    x, _ = orc.get_grids(n, length)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    box = np.where(np.abs(Z) > 6.0, 50.0, 0.0)
+    absorber = 0.8 * np.clip(r - 5.0, 0.0, None) ** 2
+    v_static = box - 1j * absorber
+    static, beam = "x**2 + y**2 + z**2", "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
+    problem = orc.Problem(n, length, "real", dt, beta, static=static, dynamic=beam, v_static=v_static)
+    solve = make(n, length, "real", dt, beta, static, beam, precision=precision)[1]
+    solve.volume_upload(_lib.TGT_VOL, v_static.real)
+    solve.volume_upload(_lib.TGT_VOL_IM, v_static.imag)
+    psi0 = orc.synthetic_state(n, length, seed=8, beta=beta)
+    solve.upload(psi0)
+    solve.step(6, midpoint_times(0, 6, dt))
+    expected = problem.run(psi0, 6)
+    assert rel_l2(solve.to_numpy(), expected) < tol
+    assert solve.norm() == pytest.approx(orc.get_N(expected, x), rel=tol * 10)
+    assert orc.get_N(expected, x) < orc.get_N(psi0, x) * (1 - 1e-5)     # the absorber removed norm
