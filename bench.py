@@ -43,6 +43,7 @@ NK_INTERVAL = 500
 HBM_PEAK_GBS = 8000.0
 PASSES_PER_STEP = 4          # steady-state psi sweeps per real-time step (fused z pass)
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_stirring256_c64.txt"
+STATS_FILE = ROOT / "profiles" / "kernel_stats_stirring256.md"
 
 
 def log(msg):
@@ -99,6 +100,19 @@ def measured_traffic(kernel):
         m = re.match(r"\s*traffic\s+(\w+)\s+([0-9.eE+]+)\s*$", line)
         if m and m.group(1) == kernel:
             return float(m.group(2))
+    return None
+
+
+def traced_avg_us(kind, tp, b, waves):
+    """Average duration (us) of the complex64 256^3 `kind` pass of variant
+    (tp, b, waves) in the committed rocprofv3 kernel-trace summary of this
+    workload, else None."""
+    if not STATS_FILE.exists():
+        return None
+    label = f"| {kind} c64 n{N} (TP={tp},B={b},W={waves}) |"
+    for line in STATS_FILE.read_text().splitlines():
+        if line.startswith(label):
+            return float(line.split("|")[3])
     return None
 
 
@@ -237,6 +251,8 @@ def main(argv=None):
     achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
     per_kernel = {kk: {"avg_us": round(v[0] / v[1] * 1e3, 3), "launches": v[1]}
                   for kk, v in timing.items() if v[1]}
+    dom_variant = _lib.pass_variants(N, _lib.C64)[solve64.variants()[_lib.PASS_KINDS.index(dominant)]]
+    dom_traced_us = traced_avg_us(dominant, *dom_variant)
     solve64.close()
 
     copy_bytes = 256 << 20
@@ -301,7 +317,12 @@ def main(argv=None):
                    "grid": N, "precision": "complex64 field, FP64 reductions/time"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
-                     "algorithmic_bytes": bytes_per_launch, "avg_us": dom_ms * 1e3},
+                     "algorithmic_bytes": bytes_per_launch, "avg_us": dom_ms * 1e3,
+                     # the same variant's average in the committed kernel trace of this workload;
+                     # the event figure above carries the events' own cost
+                     "traced_avg_us": dom_traced_us,
+                     "traced_frac": (bytes_per_launch / (dom_traced_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+                                     if dom_traced_us else None)},
         "step_roofline_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
         "stream_copy_gbs": copy_gbs,
         "per_kernel": per_kernel,
