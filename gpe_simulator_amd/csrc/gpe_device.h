@@ -43,6 +43,20 @@ __host__ __device__ __forceinline__ cpx<R> mul_i(cpx<R> a) {
   }
 }
 
+// FP64 product and sum, each rounded on its own and never fused into an FMA
+// with the operation that feeds or consumes it, as the plain C expression
+// a*b + c evaluates without contraction.  hipcc contracts by default, and
+// __dmul_rn / __dadd_rn are plain * and + in HIP's headers, so they fuse as
+// well once inlined; the pragma drops the contract flag of these operations.
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
 // a * w with the product formed in FP64 and rounded once to R: for factors
 // that are themselves computed per point in FP64 (the kinetic phase) and for
 // the generic transform's FP64 twiddle table.

@@ -329,9 +329,16 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   constexpr bool kStageKin = TF::kFast && KIND == K_X_KIN;
   constexpr bool kStagePz = TF::kFast && (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST);
   __shared__ cpx<double> kin_lds[kStageKin ? TF::N : 1];
+  __shared__ double kx2_lds[kStageKin ? TF::N : 1];   // kx*kx for the momentum cutoff
   __shared__ R pz_lds[kStagePz ? 2 * TF::N : 1];
   if constexpr (kStageKin) {
     for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) kin_lds[e] = gload(s.kin_x + e);
+    if (s.masked) {
+      for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) {
+        const double kx = gload(s.karr + e);
+        kx2_lds[e] = mul_rn(kx, kx);
+      }
+    }
   }
   if constexpr (kStagePz) {
     for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) {
@@ -342,6 +349,14 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   auto kin_at = [&](int i) -> cpx<double> {
     if constexpr (kStageKin) return kin_lds[i];
     else return gload(s.kin_x + i);
+  };
+  auto kx2_at = [&](int i) -> double {
+    if constexpr (kStageKin) {
+      return kx2_lds[i];
+    } else {
+      const double kx = gload(s.karr + i);
+      return mul_rn(kx, kx);
+    }
   };
   // Elements per scheduling chunk in the kinetic loop: the compiler may not
   // move work across chunk boundaries, which bounds the FP64 factors it keeps
@@ -403,8 +418,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         const cpx<double> eyz = cmul(gload(s.kin_y + pj), gload(s.kin_z + pk));
         double ky2 = 0.0, kz2 = 0.0;
         if (s.masked) {
-          ky2 = __dmul_rn(gload(s.karr + pj), gload(s.karr + pj));
-          kz2 = __dmul_rn(gload(s.karr + pk), gload(s.karr + pk));
+          ky2 = mul_rn(gload(s.karr + pj), gload(s.karr + pj));
+          kz2 = mul_rn(gload(s.karr + pk), gload(s.karr + pk));
         }
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -412,9 +427,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
             const int i = TF::pos(t, m);
             cpx<double> f = cmul(kin_at(i), eyz);
             if (s.masked) {
-              const double kx = gload(s.karr + i);
-              // same association as kx*kx + ky*ky + kz*kz, without contraction
-              const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), ky2), kz2);
+              // (kx*kx + ky*ky) + kz*kz with every product and sum rounded
+              const double k2 = add_rn(add_rn(kx2_at(i), ky2), kz2);
               if (k2 > s.kcut2) f = cpx<double>{0.0, 0.0};
             }
             v[m] = cmul_d(v[m], f);

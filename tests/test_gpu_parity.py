@@ -206,3 +206,38 @@ def test_complex_static_volume_on_fast_sides(n, precision, tol, require_gpu):
     assert rel_l2(solve.to_numpy(), expected) < tol
     assert solve.norm() == pytest.approx(orc.get_N(expected, x), rel=tol * 10)
     assert orc.get_N(expected, x) < orc.get_N(psi0, x) * (1 - 1e-5)     # the absorber removed norm
+
+
+@pytest.mark.parametrize("n", [18, 64, 128, 256])
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_cutoff_mask_matches_the_oracle_on_the_sphere(n, precision, require_gpu):
+    """The zeroed modes of the masked kinetic step are exactly the oracle's
+    (gpe/kernels.py:24-45: zero where kx*kx + ky*ky + kz*kz > kcut*kcut).
+
+    A delta at the origin has a flat spectrum.  With beta = 0 and no potential
+    the half-steps multiply by exactly 1, so after one step the spectrum is the
+    kinetic phase (modulus 1) on kept modes and 0 on cut ones.  The cutoff is
+    |k| of the axis mode 9 on a box of 13.7: the axis modes tie (k^2 ==
+    kcut^2, kept by the strict >), and of the other modes with integer
+    i^2 + j^2 + k^2 = 81 some round above the cutoff and some below.  Summing
+    in another order, contracting to FMAs or comparing with >= moves some of
+    them; the input check below shows that the reversed order does."""
+    length = 13.7
+    kcut = abs(orc.get_grids(n, length)[1][9])
+    problem, solve = make(n, length, "real", 1e-3, 0.0, kcut=kcut, precision=precision)
+    k = problem.k
+    k2 = orc.k_squared(k)
+    kcut2 = kcut * kcut
+    cut = k2 > kcut2
+    reversed_order = (k[None, None, :] ** 2 + k[None, :, None] ** 2) + k[:, None, None] ** 2
+    assert np.count_nonzero(k2 == kcut2) >= 3
+    assert np.count_nonzero((reversed_order > kcut2) != cut) > 0
+    assert 0 < np.count_nonzero(cut) < cut.size
+    psi0 = np.zeros((n,) * 3, dtype=np.complex128)
+    psi0[0, 0, 0] = 1.0
+    solve.upload(psi0)
+    solve.step(1)
+    spectrum = np.abs(np.fft.fftn(solve.to_numpy()))
+    solve.close()
+    assert np.all(np.abs(spectrum[~cut] - 1) < 1e-4), precision
+    assert np.all(spectrum[cut] < 1e-4), precision
