@@ -32,7 +32,8 @@ EXPORTS = (
     "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
-    "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history",
+    "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
+    "gpe_mark", "gpe_mark_elapsed",
 )
 
 
@@ -68,6 +69,7 @@ def _declare(lib):
         "gpe_ctx_create": (c_int, [c_int, POINTER(vp)]),
         "gpe_ctx_destroy": (c_int, [vp]),
         "gpe_mem_info": (c_int, [vp, POINTER(c_size_t), POINTER(c_size_t)]),
+        "gpe_device_info": (c_int, [vp, c_char_p, i32, POINTER(i32), POINTER(i32)]),
         "gpe_solve_create": (c_int, [vp, POINTER(SolveDesc), POINTER(vp)]),
         "gpe_solve_destroy": (c_int, [vp]),
         "gpe_solve_footprint": (c_int, [POINTER(SolveDesc), POINTER(c_size_t), POINTER(c_size_t)]),
@@ -85,6 +87,8 @@ def _declare(lib):
         "gpe_normalise": (c_int, [vp]),
         "gpe_diagnostics": (c_int, [vp, c_double, dp, POINTER(i64), dp, i32]),
         "gpe_sync": (c_int, [vp]),
+        "gpe_mark": (c_int, [vp, i32]),
+        "gpe_mark_elapsed": (c_int, [vp, dp]),
         "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
         "gpe_angular_momentum": (c_int, [vp, dp]),
@@ -163,6 +167,13 @@ class Context:
         v = np.ascontiguousarray(variants, dtype=np.int32)
         check(lib().gpe_ctx_set_variants(self.handle, int(n), int(precision), int(kind), i32ptr(v), v.size))
         self.variant_defaults[(int(n), int(precision), int(kind))] = [int(x) for x in v]
+
+    def device_info(self):
+        """{"name", "runtime_version", "driver_version"} of the context's device."""
+        name, rt, drv = ctypes.create_string_buffer(256), c_int32(), c_int32()
+        check(lib().gpe_device_info(self.handle, name, len(name), byref(rt), byref(drv)))
+        return dict(name=name.value.decode(errors="replace"), runtime_version=rt.value,
+                    driver_version=drv.value)
 
     def bench_copy(self, nbytes, reps=20):
         out = c_double()
@@ -335,6 +346,16 @@ class Solve:
 
     def sync(self):
         check(lib().gpe_sync(self.handle))
+
+    def mark(self, which):
+        """Record timing mark 0 (start) or 1 (stop) on the solve's stream."""
+        check(lib().gpe_mark(self.handle, int(which)))
+
+    def marked_ms(self):
+        """Milliseconds between the marks, after waiting for the stop mark."""
+        out = c_double()
+        check(lib().gpe_mark_elapsed(self.handle, byref(out)))
+        return out.value
 
     # -- reductions
     def norm(self):

@@ -232,6 +232,7 @@ struct gpe_solve {
   bool pending = false;                   // psi_true = psi * scal[1]
   bool timing = false;
   PassTimer timer;
+  hipEvent_t mark[2] = {nullptr, nullptr};  // gpe_mark start / stop
 };
 
 namespace {
@@ -685,6 +686,28 @@ int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
   return GPE_OK;
 }
 
+int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime_version,
+                    int32_t* driver_version) {
+  if (!ctx || !name || name_len < 1 || !runtime_version || !driver_version)
+    return fail(GPE_ERR_INVALID, "null argument");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  // marketing name and architecture; the architecture alone when the
+  // runtime reports no name
+  std::string text(prop.name, strnlen(prop.name, sizeof(prop.name)));
+  const std::string arch(prop.gcnArchName, strnlen(prop.gcnArchName, sizeof(prop.gcnArchName)));
+  text = text.empty() ? arch : text + " (" + arch + ")";
+  const size_t len = std::min(text.size(), (size_t)name_len - 1);
+  memcpy(name, text.data(), len);
+  name[len] = '\0';
+  int rt = 0, drv = 0;
+  HIPCHK(hipRuntimeGetVersion(&rt));
+  HIPCHK(hipDriverGetVersion(&drv));
+  *runtime_version = rt;
+  *driver_version = drv;
+  return GPE_OK;
+}
+
 int gpe_pass_variants(int32_t n, int32_t precision, int32_t* count) {
   if (!count) return fail(GPE_ERR_INVALID, "null argument");
   if (precision != GPE_C64 && precision != GPE_C128) return fail(GPE_ERR_INVALID, "bad precision");
@@ -843,6 +866,8 @@ int gpe_solve_destroy(gpe_solve* s) {
     if (s->tlist_ev[f]) (void)hipEventDestroy(s->tlist_ev[f]);
   }
   s->timer.destroy();
+  for (hipEvent_t e : s->mark)
+    if (e) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
   return GPE_OK;
@@ -1370,6 +1395,27 @@ int gpe_sync(gpe_solve* s) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   CHK(set_device(s));
   HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_mark(gpe_solve* s, int32_t which) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (which != 0 && which != 1) return fail(GPE_ERR_INVALID, "mark must be 0 (start) or 1 (stop)");
+  CHK(set_device(s));
+  if (!s->mark[which]) HIPCHK(hipEventCreate(&s->mark[which]));
+  HIPCHK(hipEventRecord(s->mark[which], s->st));
+  s->timer.chained = false;               // a pass timed next starts after the mark
+  return GPE_OK;
+}
+
+int gpe_mark_elapsed(gpe_solve* s, double* ms) {
+  if (!s || !ms) return fail(GPE_ERR_INVALID, "null argument");
+  if (!s->mark[0] || !s->mark[1]) return fail(GPE_ERR_INVALID, "both marks must be recorded first");
+  CHK(set_device(s));
+  HIPCHK(hipEventSynchronize(s->mark[1]));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, s->mark[0], s->mark[1]));
+  *ms = t;
   return GPE_OK;
 }
 

@@ -51,6 +51,13 @@ int gpe_ctx_create(int device, gpe_ctx** out);
 int gpe_ctx_destroy(gpe_ctx* ctx);
 /* Free / total device memory in bytes (serving admission). */
 int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* Device name and architecture, e.g. "<name> (gfx950:...)", or the
+ * architecture alone when the runtime reports no name (NUL-terminated,
+ * truncated to name_len bytes); HIP runtime and driver versions of the
+ * context's device: the report metadata that
+ * gpe/benchmark.py:204-233 reads from CuPy's runtime. */
+int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime_version,
+                    int32_t* driver_version);
 
 /* Pass tile variants.  A step is a sequence of passes (kinds 0..GPE_PASS_KINDS-1:
  * z_first, z_mid, z_last, y_fwd, y_inv, x_kin, z_diag, x_diag, z_fwd, y_lz,
@@ -203,6 +210,15 @@ int gpe_batch_step_timed(gpe_batch* batch, int64_t nsteps, double* ms_per_kind, 
 
 /* Wait for all work of the solve. */
 int gpe_sync(gpe_solve* solve);
+
+/* Timing marks on the solve's stream: the CuPy event pair that brackets one
+ * timed batch in gpe/benchmark.py:59-69.  gpe_mark(solve, 0) records the
+ * start mark and gpe_mark(solve, 1) the stop mark, after whatever the solve
+ * has queued by then; gpe_mark_elapsed waits for the stop mark and returns the
+ * milliseconds between the two.  Like the CuPy pair, the interval includes
+ * device idle time while the host was still submitting. */
+int gpe_mark(gpe_solve* solve, int32_t which);
+int gpe_mark_elapsed(gpe_solve* solve, double* ms);
 
 /* Measurement hooks (no reference counterpart; gpe/benchmark.py:59-69 times
  * whole batches).  gpe_step_timed runs gpe_step with a HIP event pair around
