@@ -313,7 +313,8 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "c64+f64-reductions",
         "data": "synthetic (seeded Thomas-Fermi + noise), resident in HBM",
-        "config": {"workload": "stirring run 256^3, L=40, beta=4000, dt=2.5e-4, nk every 500 steps",
+        "config": {"workload": "stirring run 256^3, L=40, beta=4000, dt=2.5e-4, nk every 500 steps "
+                               "(dk shells and the 128-bin view up to k_max)",
                    "grid": N, "precision": "complex64 field, FP64 reductions/time"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),

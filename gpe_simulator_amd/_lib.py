@@ -18,6 +18,7 @@ LIBRARY = Path(__file__).resolve().parent / "libgpe_hip.so"
 
 GPE_OK, GPE_ERR_INVALID, GPE_ERR_NOMEM, GPE_ERR_HIP, GPE_ERR_STATE = 0, -1, -2, -3, -4
 C64, C128 = 0, 1
+VIEW_BINS = 128                  # GPE_VIEW_BINS: equal-width bins of the spectrum view
 REAL, IMAG = 0, 1
 TGT_XY, TGT_Z, TGT_VOL, TGT_VOL_IM = 0, 1, 2, 3
 
@@ -33,7 +34,7 @@ EXPORTS = (
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
-    "gpe_mark", "gpe_mark_elapsed",
+    "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view",
 )
 
 
@@ -86,6 +87,7 @@ def _declare(lib):
         "gpe_norm": (c_int, [vp, dp]),
         "gpe_normalise": (c_int, [vp]),
         "gpe_diagnostics": (c_int, [vp, c_double, dp, POINTER(i64), dp, i32]),
+        "gpe_spectrum_view": (c_int, [vp, POINTER(i64), dp, i32]),
         "gpe_sync": (c_int, [vp]),
         "gpe_mark": (c_int, [vp, i32]),
         "gpe_mark_elapsed": (c_int, [vp, dp]),
@@ -383,6 +385,11 @@ class Solve:
         if spectrum:
             result["counts"] = counts
             result["occupation"] = occ
+            view_counts = np.zeros(VIEW_BINS, dtype=np.int64)
+            view_occ = np.zeros(VIEW_BINS)
+            check(lib().gpe_spectrum_view(self.handle, i64ptr(view_counts), dptr(view_occ), VIEW_BINS))
+            result["view_counts"] = view_counts
+            result["view_occupation"] = view_occ
         return result
 
 

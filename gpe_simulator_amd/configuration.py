@@ -10,7 +10,10 @@ r_abs 5, A_abs 100, dt 0.01, N_atom 150000, a 100).
 Schema 2 rejects unknown keys, so engine options (precision) are not YAML keys
 here: they are chosen with the CLI flag / environment variable documented in
 main.py, or with `schema_version: 3`, which is schema 2 plus an optional
-`simulation.precision` ("complex128" default, or "complex64").
+`simulation.precision` ("complex128" default, or "complex64") and an optional
+per-stage `save.nk_view` (false by default): when true, each nk output is
+accompanied by the 128-bin equal-width view of the same spectrum
+(nkview_<step>.npz).
 """
 
 from __future__ import annotations
@@ -176,7 +179,10 @@ def normalize_config(raw):
                                 "Choose one wavefunction snapshot interval; restart checkpoints are not supported.")
                 else:
                     old["psi_interval"] = legacy
-        stage = check.merge(raw_stage, new_stage(imaginary=index == 0), path)
+        default = new_stage(imaginary=index == 0)
+        if schema3:
+            default["save"]["nk_view"] = False
+        stage = check.merge(raw_stage, default, path)
         config["stages"].append(stage)
         _validate_stage(check, config, stage, index, path)
     if check.errors:
@@ -247,7 +253,7 @@ def _validate_stage(check, config, stage, index, path):
         check.number(stage["monitor"], key, f"{path}.monitor", minimum=0, integer=True)
     for key in SAVE_INTERVALS:
         check.number(stage["save"], key, f"{path}.save", minimum=0, integer=True)
-    for key in ("save_initial", "save_final"):
+    for key in ("save_initial", "save_final", *(("nk_view",) if "nk_view" in stage["save"] else ())):
         check.boolean(stage["save"], key, f"{path}.save")
     check.text(stage["save"], "directory", f"{path}.save")
 

@@ -58,6 +58,8 @@ static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<float>), "pass argument lay
 static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<double>), "pass argument layout");
 
 int nbins_for(int n) { return int(std::floor(std::sqrt(3.0) * (n / 2) + 0.5)) + 1; }
+// Bins of the equal-width view kept after the reference's shells.
+constexpr int kViewBins = GPE_VIEW_BINS;
 
 // Compiled pass variants.  Entries of one (precision, side) are contiguous and
 // ordered by variant index; sides without a fast transform use the generic
@@ -175,6 +177,7 @@ struct gpe_ctx {
 struct gpe_solve {
   gpe_ctx* ctx = nullptr;
   int n = 0, prec = 0, kind = 0, masked = 0, nbins = 0;
+  bool view_ready = false;                // a gpe_diagnostics call filled the view
   double dt = 0, beta = 0, kcut = 0, coef = 1, dx3 = 0;
   long long n3 = 0;
   size_t esz = 0, rsz = 0;
@@ -313,6 +316,8 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
   d.hdt = R(s->dt / 2);
   d.n = s->n;
   d.nbins = s->nbins;
+  d.nview = kViewBins;
+  d.vh = s->n / 2;
   d.imag = s->kind == GPE_IMAG ? 1 : 0;
   d.masked = s->masked;
 }
@@ -617,8 +622,8 @@ int ensure_diag(gpe_solve* s) {
   if (!s->work) {
     CHK(dzalloc(&s->work, s->n3 * s->esz, s->st));
     CHK(dzalloc(&s->d_out, 16 * sizeof(double), s->st));
-    CHK(dzalloc(&s->d_nkocc, s->nbins * sizeof(double), s->st));
-    CHK(dzalloc(&s->d_nkcnt, s->nbins * sizeof(long long), s->st));
+    CHK(dzalloc(&s->d_nkocc, (s->nbins + kViewBins) * sizeof(double), s->st));
+    CHK(dzalloc(&s->d_nkcnt, (s->nbins + kViewBins) * sizeof(long long), s->st));
     changed = true;
   }
   // partial buffers follow the blocks of the pass variants in use
@@ -630,8 +635,8 @@ int ensure_diag(gpe_solve* s) {
     s->nkocc = nullptr;
     s->nkcnt = nullptr;
     s->nk_cap = 0;
-    CHK(dzalloc(&s->nkocc, (size_t)bx * s->nbins * sizeof(double), s->st));
-    CHK(dzalloc(&s->nkcnt, (size_t)bx * s->nbins * sizeof(unsigned int), s->st));
+    CHK(dzalloc(&s->nkocc, (size_t)bx * (s->nbins + kViewBins) * sizeof(double), s->st));
+    CHK(dzalloc(&s->nkcnt, (size_t)bx * (s->nbins + kViewBins) * sizeof(unsigned int), s->st));
     s->nk_cap = bx;
     changed = true;
   }
@@ -772,7 +777,7 @@ int gpe_solve_footprint(const gpe_solve_desc* d, size_t* resident, size_t* diag)
   const size_t n = (size_t)d->n, n3 = n * n * n;
   const size_t esz = d->precision == GPE_C64 ? 8 : 16, rsz = esz / 2;
   *resident = n3 * esz + n * n * (8 + rsz) + 16 * n * 8 + (4 << 20);
-  const int nb = nbins_for(d->n);
+  const int nb = nbins_for(d->n) + kViewBins;
   *diag = n3 * esz + (n * n) * (size_t)nb * 12 + kStageElems * 16;
   return GPE_OK;
 }
@@ -1101,7 +1106,7 @@ int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, do
   CHK(pass(s, K_Y_FWD, true, 0, 0, F_IN_WORK | F_OUT_WORK));
   CHK(pass(s, K_X_DIAG, true, 0, 0, F_IN_WORK));
   launch_finish_sums(s->red, bx, 2, s->d_out + 3, s->st);
-  launch_finish_nk(s->nkocc, s->nkcnt, bx, s->nbins, s->d_nkocc, s->d_nkcnt, s->st);
+  launch_finish_nk(s->nkocc, s->nkcnt, bx, s->nbins + kViewBins, s->d_nkocc, s->d_nkcnt, s->st);
   HIPCHK(hipGetLastError());
   double h[5];
   HIPCHK(hipMemcpyAsync(h, s->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -1109,6 +1114,7 @@ int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, do
     HIPCHK(hipMemcpyAsync(occupation, s->d_nkocc, s->nbins * sizeof(double), hipMemcpyDeviceToHost, s->st));
   if (counts) HIPCHK(hipMemcpyAsync(counts, s->d_nkcnt, s->nbins * sizeof(long long), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  s->view_ready = true;
   const double n3 = double(s->n3);
   const double norm = s->dx3 * h[0];
   const double ekin = h[3] * s->dx3 / n3;
@@ -1394,6 +1400,21 @@ int gpe_batch_step_timed(gpe_batch* b, int64_t nsteps, double* ms_per_kind, int6
 int gpe_sync(gpe_solve* s) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_spectrum_view(gpe_solve* s, int64_t* counts, double* occupation, int32_t nbins) {
+  if (!s || (!counts && !occupation)) return fail(GPE_ERR_INVALID, "null argument");
+  if (nbins != kViewBins) return fail(GPE_ERR_INVALID, "nbins must be " + std::to_string(kViewBins));
+  if (!s->view_ready) return fail(GPE_ERR_STATE, "no gpe_diagnostics call has filled the view yet");
+  CHK(set_device(s));
+  if (occupation)
+    HIPCHK(hipMemcpyAsync(occupation, s->d_nkocc + s->nbins, kViewBins * sizeof(double), hipMemcpyDeviceToHost,
+                          s->st));
+  if (counts)
+    HIPCHK(hipMemcpyAsync(counts, s->d_nkcnt + s->nbins, kViewBins * sizeof(long long), hipMemcpyDeviceToHost,
+                          s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   return GPE_OK;
 }

@@ -75,7 +75,9 @@ struct SolveDev {
   R beta;
   R hdt;                   // dt/2
   int n;
-  int nbins;
+  int nbins;               // dk shells of the reference's histogram
+  int nview;               // equal-width bins of the view over [0, k_max] (after the shells)
+  int vh;                  // largest |integer frequency| on an axis: n/2
   int imag;                // 1: imaginary time
   int masked;              // 1: zero modes with k^2 > kcut^2
 };
@@ -261,6 +263,21 @@ __device__ __forceinline__ int shell_index(int p, int q, int r) {
   return m + (s > m * m + m ? 1 : 0);
 }
 
+// Bin of the equal-width view: floor(nview * |m| / (sqrt(3) * h)) for the
+// integer frequency vector m and h = the largest |frequency| on an axis, i.e.
+// |k| / k_max in nview bins with k_max the corner wavenumber.  Exact: b is the
+// largest integer with 3 h^2 b^2 <= nview^2 |m|^2 (all in 64-bit integers);
+// the corner itself (b = nview) goes to the last bin.
+__device__ __forceinline__ int view_index(int p, int q, int r, int h, int nview) {
+  const long long s = (long long)p * p + (long long)q * q + (long long)r * r;
+  const long long lhs = (long long)nview * nview * s;
+  const long long c = 3LL * h * h;
+  int b = int(sqrtf(float(lhs) / float(c)));
+  while (b > 0 && c * b * b > lhs) --b;
+  while (c * (long long)(b + 1) * (b + 1) <= lhs) ++b;
+  return b < nview ? b : nview - 1;
+}
+
 // ------------------------------------------------------------ pass kernel --
 // Tile = B pencils; one block per tile.  (A multi-tile variant that prefetched
 // the next tile into registers measured slower at every size: the live
@@ -380,7 +397,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   __shared__ double hocc[KIND == K_X_DIAG ? 1024 : 1];
   __shared__ unsigned int hcnt[KIND == K_X_DIAG ? 1024 : 1];
   if constexpr (KIND == K_X_DIAG) {
-    for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
+    for (int q = threadIdx.x; q < s.nbins + s.nview; q += blockDim.x) {
       hocc[q] = 0.0;
       hcnt[q] = 0u;
     }
@@ -542,9 +559,15 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
           acc[0] += f2 * k2;
           acc[1] += f2;
-          const int sh = shell_index(freq_index(i, n), fq, fr);
+          const int fp = freq_index(i, n);
+          const int sh = shell_index(fp, fq, fr);
           atomicAdd(&hocc[sh], f2);
           atomicAdd(&hcnt[sh], 1u);
+          if (s.nview) {
+            const int vb = s.nbins + view_index(fp, fq, fr, s.vh, s.nview);
+            atomicAdd(&hocc[vb], f2);
+            atomicAdd(&hcnt[vb], 1u);
+          }
         }
       }
       write = false;
@@ -594,9 +617,10 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       s.red[bid * 2 + 1] = part[1];
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < s.nbins; q += blockDim.x) {
-      s.nkocc[(long long)bid * s.nbins + q] = hocc[q];
-      s.nkcnt[(long long)bid * s.nbins + q] = hcnt[q];
+    const int nb = s.nbins + s.nview;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+      s.nkocc[(long long)bid * nb + q] = hocc[q];
+      s.nkcnt[(long long)bid * nb + q] = hcnt[q];
     }
   }
 }

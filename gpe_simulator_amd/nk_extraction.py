@@ -2,7 +2,12 @@
 
 The device bins every mode of fftn(psi) by shell floor(|k|/dk + 1/2) with an
 integer-exact index and sums |F|^2 per shell in FP64; this module turns the
-per-shell sums into the reference's output dictionary."""
+per-shell sums into the reference's output dictionary.
+
+The same passes also bin the spectrum into VIEW_BINS (128) equal-width bins
+over [0, k_max], k_max = sqrt(3) (n/2) dk the corner wavenumber, again with
+an integer-exact index: a fixed-size view for comparing grids of different
+sides (no reference counterpart)."""
 
 from __future__ import annotations
 
@@ -10,6 +15,7 @@ import math
 
 import numpy as np
 
+from ._lib import VIEW_BINS
 from .field import DeviceField
 
 
@@ -37,3 +43,25 @@ def nk_extraction(psi, x_array, k_array, t=0.0):
         raise TypeError("psi must be a DeviceField")
     diag = psi.solve.diagnostics(t, spectrum=True)
     return shell_summary(diag["counts"], diag["occupation"], x_array, k_array, psi.solve.n)
+
+
+def view_summary(counts, occupation, x_array, k_array, n):
+    """Output fields of the equal-width view: bin centres k, edges, k_max,
+    counts, Nk = sum |F|^2 dx^3/N^3 and nk = Nk / (count dk^3), the shell
+    normalisation of the reference applied to these bins."""
+    dx = float(x_array[1] - x_array[0])
+    dk = abs(float(k_array[1] - k_array[0]))
+    counts = np.asarray(counts, dtype=np.int64)
+    Nk = np.asarray(occupation, dtype=np.float64) * (dx ** 3 / n ** 3)
+    k_max = math.sqrt(3) * (n // 2) * dk
+    width = k_max / VIEW_BINS
+    return dict(k=(np.arange(VIEW_BINS) + 0.5) * width, edges=np.arange(VIEW_BINS + 1) * width,
+                k_max=k_max, nk=Nk / (np.maximum(counts, 1) * dk ** 3), Nk=Nk, counts=counts)
+
+
+def nk_view(psi, x_array, k_array, t=0.0):
+    """The 128-bin equal-width view of |fftn(psi)|^2 (see view_summary)."""
+    if not isinstance(psi, DeviceField):
+        raise TypeError("psi must be a DeviceField")
+    diag = psi.solve.diagnostics(t, spectrum=True)
+    return view_summary(diag["view_counts"], diag["view_occupation"], x_array, k_array, psi.solve.n)

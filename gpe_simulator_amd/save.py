@@ -15,7 +15,7 @@ import numpy as np
 
 from .field import asnumpy
 from .helper import is_due
-from .nk_extraction import shell_summary
+from .nk_extraction import shell_summary, view_summary
 from .state_io import save_wavefunction
 
 TOF_FIELDS = ("show_TOF_image_interval", "show_verticle_TOF_interval", "show_TOF_slice_interval",
@@ -62,6 +62,11 @@ def save_outputs(stage, step, *, initial=False, final=False):
     if spectrum_due:
         save_npz("nk", **shell_summary(diag["counts"], diag["occupation"], stage.x_array, stage.k_array,
                                        stage.N_grids))
+        if save.get("nk_view"):
+            # the equal-width 128-bin view of the same spectrum (schema 3
+            # only; the name does not match the reference's nk_* files)
+            save_npz("nkview", **view_summary(diag["view_counts"], diag["view_occupation"], stage.x_array,
+                                              stage.k_array, stage.N_grids))
 
     images = any(due[name] for name in TOF_FIELDS) or due["show_magnifier_interval"] \
         or due["show_magnifier_slice_interval"] or due["show_phase_interval"]

@@ -66,6 +66,8 @@ int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime
  * times the variants and sets the fastest per kind.  No reference counterpart:
  * the reference's FFT plans are cuFFT's (gpe/make_propagator.py:50,52). */
 enum { GPE_PASS_KINDS = 11 };
+/* Bins of the equal-width spectrum view (gpe_spectrum_view). */
+enum { GPE_VIEW_BINS = 128 };
 /* Number of compiled variants for (side, precision); 1 for sides without a
  * fast transform. */
 int gpe_pass_variants(int32_t n, int32_t precision, int32_t* count);
@@ -171,6 +173,16 @@ int gpe_normalise(gpe_solve* solve);
  * t is the time at which the time-dependent potential enters E_pot. */
 int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts, double* occupation,
                     int32_t nbins);
+/* The same spectrum in GPE_VIEW_BINS equal-width bins over [0, k_max], with
+ * k_max = sqrt(3) * (n/2) * dk the grid's corner wavenumber: mode m (integer
+ * frequencies) goes to bin floor(GPE_VIEW_BINS * |m| / (sqrt(3) * (n/2))),
+ * the corner to the last bin, decided with exact integer arithmetic.
+ * counts / occupation as in gpe_diagnostics (nbins must be GPE_VIEW_BINS),
+ * from the most recent gpe_diagnostics call on the solve.  No reference
+ * counterpart: the reference bins by dk shells only
+ * (gpe/nk_extraction.py:16-45); this is the fixed-size view of that same
+ * |F|^2 for comparing runs of different grid sides. */
+int gpe_spectrum_view(gpe_solve* solve, int64_t* counts, double* occupation, int32_t nbins);
 
 /* Lz = Re int psi^* (x p_y - y p_x) psi dV with spectral derivatives, FP64.
  * Reference: get_L (gpe/helper.py:143-156). */

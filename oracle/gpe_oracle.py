@@ -13,7 +13,9 @@ Pins (tests/test_oracle.py): the reference's own closed-form tests
 (tests/test_propagators.py:61-158, tests/test_diagnostics.py:31-81), exact
 time reversibility, Parseval, a brute-force DFT at N <= 6, the harmonic
 ground-state energy E = mu = 3, plane-wave mu and E, the integer shell
-identity and the stage clock t = i*dt (tests/test_workflow.py:262).
+identity and the stage clock t = i*dt (tests/test_workflow.py:262).  The
+equal-width spectrum view (no reference counterpart) is pinned against a
+50-digit decimal evaluation of its defining formula.
 """
 
 from __future__ import annotations
@@ -254,6 +256,53 @@ def shell_index_integer(p, q, r):
     m = np.where(m * m > s, m - 1, m)
     m = np.where((m + 1) * (m + 1) <= s, m + 1, m)
     return m + (s > m * m + m)
+
+
+VIEW_BINS = 128
+
+
+def _isqrt(x):
+    """floor(sqrt(x)) of a non-negative int64 array, exact."""
+    m = np.floor(np.sqrt(x.astype(np.float64))).astype(np.int64)
+    m = np.where(m * m > x, m - 1, m)
+    return np.where((m + 1) * (m + 1) <= x, m + 1, m)
+
+
+def spectrum_view_bin(p, q, r, n, nview=VIEW_BINS):
+    """Bin of the equal-width spectrum view (no reference counterpart; the
+    reference bins by dk shells, gpe/nk_extraction.py:16-45): with integer
+    frequencies m = (p, q, r) and h = n // 2, bin floor(nview |m| / (sqrt(3) h)),
+    i.e. |k| / k_max in nview bins, k_max = sqrt(3) h dk the corner; the corner
+    itself goes to the last bin.  Evaluated as isqrt(floor(X)) with
+    X = nview^2 |m|^2 / (3 h^2), since floor(sqrt(X)) = isqrt(floor(X))."""
+    s = (np.asarray(p, dtype=np.int64) ** 2 + np.asarray(q, dtype=np.int64) ** 2
+         + np.asarray(r, dtype=np.int64) ** 2)
+    return view_bin_from_square(s, n, nview)
+
+
+def view_bin_from_square(s, n, nview=VIEW_BINS):
+    """spectrum_view_bin as a function of the integer |m|^2 = s."""
+    h = n // 2
+    b = _isqrt((nview * nview * np.asarray(s, dtype=np.int64)) // (3 * h * h))
+    return np.minimum(b, nview - 1)
+
+
+def spectrum_view(psi_k, x, k, nview=VIEW_BINS):
+    """|F|^2 of raw fftn(psi) in the equal-width view: counts, raw sums and the
+    shell-style normalisations Nk = sum dx^3/N^3, nk = Nk / (count dk^3)."""
+    n = psi_k.shape[0]
+    dx = float(x[1] - x[0])
+    dk = abs(float(k[1] - k[0]))
+    m = np.fft.fftfreq(n, 1.0 / n).round().astype(np.int64)
+    b = spectrum_view_bin(m[:, None, None], m[None, :, None], m[None, None, :], n, nview).ravel()
+    f2 = (psi_k.real ** 2 + psi_k.imag ** 2).ravel()
+    counts = np.bincount(b, minlength=nview)
+    occupation = np.bincount(b, weights=f2, minlength=nview)
+    Nk = occupation * (dx ** 3 / psi_k.size)
+    k_max = math.sqrt(3) * (n // 2) * dk
+    width = k_max / nview
+    return dict(counts=counts, occupation=occupation, Nk=Nk, nk=Nk / (np.maximum(counts, 1) * dk ** 3),
+                k=(np.arange(nview) + 0.5) * width, edges=np.arange(nview + 1) * width, k_max=k_max)
 
 
 def synthetic_state(n, length, seed=0, beta=1.0, potential=None):

@@ -224,11 +224,21 @@ def test_unsafe_duplicate_unknown_rejected(source):
         load_yaml(source)
 
 
-def test_schema3_adds_precision_only():
+def test_schema3_adds_precision_and_nk_view_only():
     result = load_yaml("schema_version: 3\nsimulation: {precision: complex64}\nstages: [{}]")
     assert result["simulation"]["precision"] == "complex64"
     with pytest.raises(ConfigError):
         load_yaml("schema_version: 3\nsimulation: {precision: half}\nstages: [{}]")
     default = load_yaml("schema_version: 3\nstages: [{}]")
     assert default["simulation"]["precision"] == "complex128"
+    assert default["stages"][0]["save"]["nk_view"] is False
     assert load_yaml(dump_yaml(result)) == result
+    view = load_yaml("schema_version: 3\nstages: [{save: {nk_view: true, nk_interval: 5}}]")
+    assert view["stages"][0]["save"]["nk_view"] is True
+    assert load_yaml(dump_yaml(view)) == view
+    with pytest.raises(ConfigError, match="nk_view must be true or false"):
+        load_yaml("schema_version: 3\nstages: [{save: {nk_view: 1}}]")
+    # schema 2 keeps the reference's keys only
+    assert "nk_view" not in load_yaml("schema_version: 2\nstages: [{}]")["stages"][0]["save"]
+    with pytest.raises(ConfigError, match="Unknown setting: stages.0.save.nk_view"):
+        load_yaml("schema_version: 2\nstages: [{save: {nk_view: true}}]")

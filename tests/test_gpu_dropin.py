@@ -205,3 +205,36 @@ def test_yaml_run_with_two_stages_and_all_saved_diagnostics(env):
         with scalar.open() as file:
             assert [row["step"] for row in csv.DictReader(file)] == ["1", "2"]
         assert len(list(scalar.parent.glob("*_00000002.npz"))) == 8
+
+
+def test_schema3_nk_view_output(env):
+    """save.nk_view (schema 3) writes the 128-bin view beside every nk file;
+    without it the reference's file set is unchanged."""
+    from gpe_simulator_amd.configuration import dump_yaml
+    from gpe_simulator_amd.main import main
+    env.config["schema_version"] = 3
+    env.config["simulation"]["N_grids"] = 16
+    settings = env.config["stages"][0]
+    settings["evolution"].update(steps=2)
+    settings["save"].update(nk_interval=1, nk_view=True)
+    second = json.loads(json.dumps(settings))
+    second["name"] = "plain"
+    second["initial_state"]["type"] = "continue"
+    second["save"]["nk_view"] = False
+    env.config["stages"].append(second)
+    path = env.root / "view.yaml"
+    path.write_text(dump_yaml(env.config))
+    with patch("builtins.print"):
+        main([str(path)])
+    folders = sorted((env.root / "monitor").glob("*/stage_*"))
+    assert len(folders) == 2
+    views = sorted(folders[0].glob("nkview_*.npz"))
+    assert [v.name for v in views] == ["nkview_00000001.npz", "nkview_00000002.npz"]
+    assert len(list(folders[0].glob("nk_*.npz"))) == 2
+    for view in views:
+        with np.load(view) as data:
+            assert data["counts"].sum() == 16 ** 3 and len(data["counts"]) == 128
+            assert data["Nk"].sum() == pytest.approx(1.0, rel=1e-10)      # imaginary time: unit norm
+            assert data["edges"][-1] == pytest.approx(float(data["k_max"]), rel=1e-15)
+    assert not list(folders[1].glob("nkview_*.npz"))
+    assert len(list(folders[1].glob("nk_*.npz"))) == 2

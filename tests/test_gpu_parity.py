@@ -112,6 +112,33 @@ def test_diagnostics(n, precision, require_gpu):
     dx = problem.x[1] - problem.x[0]
     np.testing.assert_allclose(got["occupation"] * dx ** 3 / n ** 3, spectrum["Nk"], rtol=tol * 10,
                                atol=tol * 1e-3)
+    view = orc.spectrum_view(np.fft.fftn(psi0), problem.x, problem.k)
+    np.testing.assert_array_equal(got["view_counts"], view["counts"])
+    np.testing.assert_allclose(got["view_occupation"] * dx ** 3 / n ** 3, view["Nk"], rtol=tol * 10,
+                               atol=tol * 1e-3)
+
+
+@pytest.mark.parametrize("n", [7, 128, 192, 256])
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_spectrum_view_and_shells_on_large_and_odd_sides(n, precision, require_gpu):
+    """Shell and equal-width view counts bit-exact, sums to FP64 (complex128)
+    or float (complex64) accuracy, on the fast sides and an odd one."""
+    length = 16.0
+    problem, solve = make(n, length, "real", 1e-3, 10.0, precision=precision)
+    psi0 = random_state(n, length, seed=11)
+    solve.upload(psi0)
+    got = solve.diagnostics(0.0)
+    solve.close()
+    psi_k = np.fft.fftn(psi0)
+    dx = problem.x[1] - problem.x[0]
+    tol = 1e-11 if precision == "c128" else 2e-6
+    shells = orc.nk_extraction(psi_k, problem.x, problem.k)
+    view = orc.spectrum_view(psi_k, problem.x, problem.k)
+    np.testing.assert_array_equal(got["counts"], shells["counts"])
+    np.testing.assert_array_equal(got["view_counts"], view["counts"])
+    assert got["view_counts"].sum() == n ** 3
+    for occ, ref in ((got["occupation"], shells["Nk"]), (got["view_occupation"], view["Nk"])):
+        np.testing.assert_allclose(occ * dx ** 3 / n ** 3, ref, rtol=tol * 10, atol=tol * 1e-3 * ref.max())
 
 
 @pytest.mark.parametrize("n", [18, 64])

@@ -250,3 +250,70 @@ def test_imaginary_time_lowers_energy_monotonically():
         psi = p.run(psi, 10)
         energies.append(p.energies(psi)["E"])
     assert all(b < a for a, b in zip(energies, energies[1:]))
+
+
+def decimal_view_bin(s, h, nview=orc.VIEW_BINS):
+    """floor(nview sqrt(s) / (sqrt(3) h)) to 50 digits, clamped to the last bin.
+    Where the value is an integer (3 h^2 b^2 == nview^2 s) the decimal digits
+    cannot settle the floor, so the exact equality decides."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 50
+    value = Decimal(nview) * Decimal(s).sqrt() / (Decimal(3).sqrt() * h)
+    nearest = int(value.to_integral_value())
+    if abs(value - nearest) < Decimal("1e-40"):
+        assert 3 * h * h * nearest * nearest == nview * nview * s
+        b = nearest
+    else:
+        b = int(value)          # floor of a positive value
+    return min(b, nview - 1)
+
+
+@pytest.mark.parametrize("n", [2, 3, 7, 8, 18, 64, 96])
+def test_spectrum_view_bin_every_square(n):
+    """Every |m|^2 the grid holds, against the defining formula."""
+    h = n // 2
+    s = np.arange(3 * h * h + 1)
+    got = orc.view_bin_from_square(s, n)
+    expected = np.array([decimal_view_bin(int(v), h) for v in s])
+    np.testing.assert_array_equal(got, expected)
+
+
+@pytest.mark.parametrize("n", [128, 192, 256, 1024])
+def test_spectrum_view_bin_at_every_boundary(n):
+    """Around each bin boundary |m|^2 = 3 h^2 b^2 / nview^2 (and at random
+    squares) at large sides, against the defining formula."""
+    h, nview = n // 2, orc.VIEW_BINS
+    rng = np.random.default_rng(n)
+    squares = set(rng.integers(0, 3 * h * h + 1, size=300).tolist()) | {0, 3 * h * h}
+    for b in range(1, nview + 1):
+        edge = 3 * h * h * b * b // (nview * nview)
+        squares |= {v for v in range(edge - 2, edge + 3) if 0 <= v <= 3 * h * h}
+    s = np.array(sorted(squares))
+    np.testing.assert_array_equal(orc.view_bin_from_square(s, n),
+                                  [decimal_view_bin(int(v), h) for v in s])
+
+
+@pytest.mark.parametrize("n", [7, 8, 18])
+def test_spectrum_view_totals_and_limits(n):
+    x, k = orc.get_grids(n, 20.0)
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=(n,) * 3) + 1j * rng.normal(size=(n,) * 3)
+    view = orc.spectrum_view(np.fft.fftn(psi), x, k)
+    shells = orc.nk_extraction(np.fft.fftn(psi), x, k)
+    assert view["counts"].sum() == n ** 3
+    assert view["Nk"].sum() == pytest.approx(orc.get_N(psi, x), rel=1e-12)
+    assert view["Nk"].sum() == pytest.approx(shells["Nk"].sum(), rel=1e-12)
+    dk = abs(k[1] - k[0])
+    assert view["k_max"] == pytest.approx(math.sqrt(3) * (n // 2) * dk, rel=1e-15)
+    assert view["edges"][-1] == pytest.approx(view["k_max"], rel=1e-15)
+    h = n // 2
+    assert orc.spectrum_view_bin(0, 0, 0, n) == 0
+    assert orc.spectrum_view_bin(h, h, -h, n) == orc.VIEW_BINS - 1
+    # a plane wave lands in the bin of its |m|
+    mode = (1, 2, -2)
+    wave = np.exp(1j * (mode[0] * x[:, None, None] + mode[1] * x[None, :, None] + mode[2] * x[None, None, :])
+                  * TWO_PI / 20.0)
+    view = orc.spectrum_view(np.fft.fftn(wave), x, k)
+    expected = orc.spectrum_view_bin(*mode, n)
+    assert view["occupation"][expected] == pytest.approx(np.sum(np.abs(np.fft.fftn(wave)) ** 2), rel=1e-12)
+    assert view["occupation"].sum() - view["occupation"][expected] < 1e-12 * view["occupation"][expected]
