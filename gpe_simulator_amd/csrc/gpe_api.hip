@@ -1104,9 +1104,10 @@ int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, do
   launch_finish_sums(s->red, bz, 3, s->d_out, s->st);
   HIPCHK(hipGetLastError());
   CHK(pass(s, K_Y_FWD, true, 0, 0, F_IN_WORK | F_OUT_WORK));
-  CHK(pass(s, K_X_DIAG, true, 0, 0, F_IN_WORK));
+  const bool spectrum = counts || occupation;
+  CHK(pass(s, K_X_DIAG, true, 0, 0, F_IN_WORK | (spectrum ? F_SPECTRUM : 0)));
   launch_finish_sums(s->red, bx, 2, s->d_out + 3, s->st);
-  launch_finish_nk(s->nkocc, s->nkcnt, bx, s->nbins + kViewBins, s->d_nkocc, s->d_nkcnt, s->st);
+  if (spectrum) launch_finish_nk(s->nkocc, s->nkcnt, bx, s->nbins + kViewBins, s->d_nkocc, s->d_nkcnt, s->st);
   HIPCHK(hipGetLastError());
   double h[5];
   HIPCHK(hipMemcpyAsync(h, s->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, s->st));
@@ -1114,7 +1115,7 @@ int gpe_diagnostics(gpe_solve* s, double t, double* scalars, int64_t* counts, do
     HIPCHK(hipMemcpyAsync(occupation, s->d_nkocc, s->nbins * sizeof(double), hipMemcpyDeviceToHost, s->st));
   if (counts) HIPCHK(hipMemcpyAsync(counts, s->d_nkcnt, s->nbins * sizeof(long long), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  s->view_ready = true;
+  if (spectrum) s->view_ready = true;
   const double n3 = double(s->n3);
   const double norm = s->dx3 * h[0];
   const double ekin = h[3] * s->dx3 / n3;

@@ -45,6 +45,7 @@ enum PassFlags : int {
   F_IN_WORK = 2,   // read from the workspace instead of psi
   F_OUT_WORK = 4,  // write to the workspace instead of psi
   F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
+  F_SPECTRUM = 16, // X_DIAG: also bin |F|^2 into the shells and the view
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -266,15 +267,21 @@ __device__ __forceinline__ int shell_index(int p, int q, int r) {
 // Bin of the equal-width view: floor(nview * |m| / (sqrt(3) * h)) for the
 // integer frequency vector m and h = the largest |frequency| on an axis, i.e.
 // |k| / k_max in nview bins with k_max the corner wavenumber.  Exact: b is the
-// largest integer with 3 h^2 b^2 <= nview^2 |m|^2 (all in 64-bit integers);
-// the corner itself (b = nview) goes to the last bin.
-__device__ __forceinline__ int view_index(int p, int q, int r, int h, int nview) {
-  const long long s = (long long)p * p + (long long)q * q + (long long)r * r;
-  const long long lhs = (long long)nview * nview * s;
-  const long long c = 3LL * h * h;
-  int b = int(sqrtf(float(lhs) / float(c)));
-  while (b > 0 && c * b * b > lhs) --b;
-  while (c * (long long)(b + 1) * (b + 1) <= lhs) ++b;
+// largest integer with 3 h^2 b^2 <= nview^2 |m|^2.  The float estimate
+// sqrt(|m|^2) * scale, scale = nview / (sqrt(3) h), is off by at most one (a
+// few ulp on a value below nview + 1), and one correction step settles it with
+// FP64 products of integers below 2^53, which are exact.  The corner itself
+// (b = nview) goes to the last bin.
+__device__ __forceinline__ int view_index(int p, int q, int r, int h, int nview, float scale) {
+  const int s = p * p + q * q + r * r;
+  const double lhs = double(nview * nview) * double(s);
+  const double c = 3.0 * double(h * h);
+  int b = int(__builtin_amdgcn_sqrtf(float(s)) * scale);
+  if (c * double(b * b) > lhs) {
+    --b;
+  } else if (c * double((b + 1) * (b + 1)) <= lhs) {
+    ++b;
+  }
   return b < nview ? b : nview - 1;
 }
 
@@ -396,7 +403,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   double acc[3] = {0.0, 0.0, 0.0};
   __shared__ double hocc[KIND == K_X_DIAG ? 1024 : 1];
   __shared__ unsigned int hcnt[KIND == K_X_DIAG ? 1024 : 1];
-  if constexpr (KIND == K_X_DIAG) {
+  const bool spectrum = KIND == K_X_DIAG && (a.flags & F_SPECTRUM) != 0;
+  if (spectrum) {
     for (int q = threadIdx.x; q < s.nbins + s.nview; q += blockDim.x) {
       hocc[q] = 0.0;
       hcnt[q] = 0u;
@@ -550,6 +558,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       TF::template fft<-1>(v, lp, t, twp, n);
       const double ky = gload(s.karr + pj), kz = gload(s.karr + pk);
       const int fq = freq_index(pj, n), fr = freq_index(pk, n);
+      const float view_scale = s.nview ? float(s.nview) / (1.7320508f * float(s.vh)) : 0.f;
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (TF::valid(t, m, n)) {
@@ -559,14 +568,16 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           const double f2 = double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
           acc[0] += f2 * k2;
           acc[1] += f2;
-          const int fp = freq_index(i, n);
-          const int sh = shell_index(fp, fq, fr);
-          atomicAdd(&hocc[sh], f2);
-          atomicAdd(&hcnt[sh], 1u);
-          if (s.nview) {
-            const int vb = s.nbins + view_index(fp, fq, fr, s.vh, s.nview);
-            atomicAdd(&hocc[vb], f2);
-            atomicAdd(&hcnt[vb], 1u);
+          if (spectrum) {
+            const int fp = freq_index(i, n);
+            const int sh = shell_index(fp, fq, fr);
+            atomicAdd(&hocc[sh], f2);
+            atomicAdd(&hcnt[sh], 1u);
+            if (s.nview) {
+              const int vb = s.nbins + view_index(fp, fq, fr, s.vh, s.nview, view_scale);
+              atomicAdd(&hocc[vb], f2);
+              atomicAdd(&hcnt[vb], 1u);
+            }
           }
         }
       }
@@ -616,8 +627,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       s.red[bid * 2 + 0] = part[0];
       s.red[bid * 2 + 1] = part[1];
     }
-    __syncthreads();
-    const int nb = s.nbins + s.nview;
+    const int nb = spectrum ? s.nbins + s.nview : 0;
+    if (nb) __syncthreads();
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
       s.nkocc[(long long)bid * nb + q] = hocc[q];
       s.nkcnt[(long long)bid * nb + q] = hcnt[q];

@@ -168,7 +168,9 @@ int gpe_normalise(gpe_solve* solve);
  *   scalars[5] mu        (E_kin + E_pot + 2 E_int) / norm
  *   scalars[6] parseval  dx^3/N^3 sum |F|^2
  * counts / occupation: per shell floor(|k|/dk + 1/2), nbins values, raw
- * sum |F|^2 (multiply by dx^3/N^3 for Nk).  Reference: save_outputs'
+ * sum |F|^2 (multiply by dx^3/N^3 for Nk).  With both null the modes are not
+ * binned at all (scalars only; gpe_spectrum_view keeps the view of the last
+ * call that binned them).  Reference: save_outputs'
  * shared fftn (gpe/save.py:31-57) and nk_extraction (gpe/nk_extraction.py:16-45).
  * t is the time at which the time-dependent potential enters E_pot. */
 int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts, double* occupation,
@@ -178,7 +180,8 @@ int gpe_diagnostics(gpe_solve* solve, double t, double* scalars, int64_t* counts
  * frequencies) goes to bin floor(GPE_VIEW_BINS * |m| / (sqrt(3) * (n/2))),
  * the corner to the last bin, decided with exact integer arithmetic.
  * counts / occupation as in gpe_diagnostics (nbins must be GPE_VIEW_BINS),
- * from the most recent gpe_diagnostics call on the solve.  No reference
+ * from the most recent gpe_diagnostics call on the solve that asked for the
+ * spectrum (non-null counts or occupation).  No reference
  * counterpart: the reference bins by dk shells only
  * (gpe/nk_extraction.py:16-45); this is the fixed-size view of that same
  * |F|^2 for comparing runs of different grid sides. */

@@ -268,3 +268,28 @@ def test_cutoff_mask_matches_the_oracle_on_the_sphere(n, precision, require_gpu)
     solve.close()
     assert np.all(np.abs(spectrum[~cut] - 1) < 1e-4), precision
     assert np.all(spectrum[cut] < 1e-4), precision
+
+
+def test_scalar_only_diagnostics_keep_the_last_spectrum(require_gpu):
+    """Diagnostics without the spectrum return the same scalars and leave the
+    view of the last spectrum call in place."""
+    from gpe_simulator_amd import _lib
+    length, n = 12.0, 64
+    problem, solve = make(n, length, "real", 1e-3, 30.0, "x**2 + y**2 + z**2")
+    with pytest.raises(_lib.GpeError, match="no gpe_diagnostics call"):
+        out = np.zeros(_lib.VIEW_BINS, dtype=np.int64)
+        _lib.check(_lib.lib().gpe_spectrum_view(solve.handle, _lib.i64ptr(out), None, _lib.VIEW_BINS))
+    psi0 = random_state(n, length, seed=5)
+    solve.upload(psi0)
+    first = solve.diagnostics(0.0)
+    solve.step(3)
+    scalars = solve.diagnostics(0.0, spectrum=False)
+    again = solve.diagnostics(0.0)
+    for key in ("norm", "E_kin", "E_pot", "E_int", "mu"):
+        assert scalars[key] == again[key], key
+    out = np.zeros(_lib.VIEW_BINS, dtype=np.int64)
+    occ = np.zeros(_lib.VIEW_BINS)
+    _lib.check(_lib.lib().gpe_spectrum_view(solve.handle, _lib.i64ptr(out), _lib.dptr(occ), _lib.VIEW_BINS))
+    np.testing.assert_array_equal(occ, again["view_occupation"])
+    assert not np.array_equal(first["view_occupation"], again["view_occupation"])
+    solve.close()
