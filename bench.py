@@ -10,6 +10,8 @@ the state is a seeded synthetic Thomas-Fermi profile plus noise, resident in
 HBM before timing starts.
 
 Beside it, the same line reports: the complex128 run of the same workload,
+the same complex128 run with the reference's launch structure (one launch per
+reference kernel and transform axis, 9 sweeps of psi per step instead of 4),
 per-kernel HIP-event timings, the oracle on the host as the CPU baseline, a
 short seeded serving stream (p50/p99 time-to-result), the 64-solve 96^3
 batched ground-state sweep against sequential complex128 solves, and a
@@ -63,19 +65,22 @@ def synthetic_state(x, seed=0):
     return psi
 
 
-def run_engine(precision, psi0, x, k, steps, warmup):
+def run_engine(precision, psi0, x, k, steps, warmup, unfused=False):
+    """Timed steps of the workload; `unfused` runs the reference-structured
+    step (one launch per reference kernel and axis) instead of the fused plan."""
     from gpe_simulator_amd.engine import make_solve, midpoint_times
     solve = make_solve(N, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=precision)
     solve.upload(psi0)
     tm = midpoint_times(0, warmup + steps, DT)
-    solve.step(warmup, tm[:warmup])
+    step = solve.step_unfused if unfused else solve.step
+    step(warmup, tm[:warmup])
     solve.diagnostics(warmup * DT)
     solve.sync()
     done = 0
     t0 = time.perf_counter()
     while done < steps:
         chunk = min(NK_INTERVAL, steps - done)
-        solve.step(chunk, tm[warmup + done: warmup + done + chunk])
+        step(chunk, tm[warmup + done: warmup + done + chunk])
         done += chunk
         if done % NK_INTERVAL == 0:
             solve.diagnostics((warmup + done) * DT)      # nk histogram + FP64 scalars (syncs)
@@ -222,6 +227,8 @@ def main(argv=None):
     parser.add_argument("--cpu-samples", type=int, default=3)
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-c128", action="store_true")
+    parser.add_argument("--unfused-steps", type=int, default=500,
+                        help="steps of the reference-structured c128 run (at most the c128 steps)")
     parser.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving stream")
     parser.add_argument("--sweep-count", type=int, default=64, help="0 skips the batched sweep")
     parser.add_argument("--stir-count", type=int, default=16, help="0 skips the stirring-speed sweep")
@@ -270,6 +277,16 @@ def main(argv=None):
         if c128_steps == args.steps:
             acc = accuracy(psi0, x, final64, (solve128.to_numpy(), diag128))
         solve128.close()
+        unfused_steps = min(c128_steps, args.unfused_steps)
+        log(f"reference-structured c128 run: {unfused_steps} steps")
+        solve_u, el_u, diag_u = run_engine("c128", psi0, x, k, unfused_steps, args.warmup, unfused=True)
+        solve_u.close()
+        vu = N ** 3 * unfused_steps / el_u
+        result_c128["unfused"] = {
+            "value": vu, "ms_per_step": el_u * 1e3 / unfused_steps, "steps": unfused_steps,
+            "launches_per_step": 9, "bytes_per_point": 9 * 2 * 16,
+            "hbm_frac": vu * 9 * 2 * 16 / (HBM_PEAK_GBS * 1e9),
+            "speedup_fused_c128": v128 / vu, "speedup_c64": value / vu}
     del final64
 
     cpu, cpu_threads = None, None

@@ -34,7 +34,7 @@ EXPORTS = (
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
-    "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view",
+    "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused",
 )
 
 
@@ -83,6 +83,7 @@ def _declare(lib):
         "gpe_volume_fill": (c_int, [vp, i32, c_double]),
         "gpe_volume_upload": (c_int, [vp, i32, dp, i64, i64, i32]),
         "gpe_step": (c_int, [vp, i64, dp]),
+        "gpe_step_unfused": (c_int, [vp, i64, dp]),
         "gpe_norm_history": (c_int, [vp, dp, i64]),
         "gpe_norm": (c_int, [vp, dp]),
         "gpe_normalise": (c_int, [vp]),
@@ -319,6 +320,16 @@ class Solve:
             if tm.size < nsteps:
                 raise ValueError("t_mid needs one time per step")
         check(lib().gpe_step(self.handle, int(nsteps), dptr(tm) if tm is not None else None))
+
+    def step_unfused(self, nsteps=1, t_mid=None):
+        """Real-time steps with the reference's launch structure (9 sweeps of
+        psi per step instead of 4); same results to rounding."""
+        tm = None
+        if t_mid is not None:
+            tm = np.ascontiguousarray(t_mid, dtype=np.float64)
+            if tm.size < nsteps:
+                raise ValueError("t_mid needs one time per step")
+        check(lib().gpe_step_unfused(self.handle, int(nsteps), dptr(tm) if tm is not None else None))
 
     def step_timed(self, nsteps, t_mid=None):
         tm = None if t_mid is None else np.ascontiguousarray(t_mid, dtype=np.float64)

@@ -174,6 +174,13 @@ struct gpe_ctx {
   std::map<std::tuple<int, int, int>, std::array<int, kPassKinds>> variants;
 };
 
+// Pass kinds outside the tuned set (the reference-structured step) run the
+// variant chosen for the tuned kind along the same axis.
+int tuned_kind(int kind) {
+  if (kind < kPassKinds) return kind;
+  return kind == K_Z_HALF || kind == K_Z_INV ? K_Z_FIRST : K_X_KIN;
+}
+
 struct gpe_solve {
   gpe_ctx* ctx = nullptr;
   int n = 0, prec = 0, kind = 0, masked = 0, nbins = 0;
@@ -185,10 +192,10 @@ struct gpe_solve {
   const PassVariant* vt = nullptr;        // compiled variants of this (precision, side)
   int nvt = 0;
   int var[kPassKinds] = {0};              // variant used by each pass kind
-  long long blocks(int kind, int n) const { return vt[var[kind]].blocks(kind, n); }
+  long long blocks(int kind, int n) const { return vt[var[tuned_kind(kind)]].blocks(kind, n); }
   hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) const {
-    return vt[var[kind]].launch(kind, args, n, batch, stream, t0, t1);
+    return vt[var[tuned_kind(kind)]].launch(kind, args, n, batch, stream, t0, t1);
   }
 
   void* psi = nullptr;
@@ -350,7 +357,7 @@ int ensure_red(gpe_solve* s, long long need) {
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index};
   hipEvent_t stop = nullptr;
-  if (s->timing) HIPCHK(s->timer.begin(kind, s->st, &stop));
+  if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &stop));
   hipError_t e = s->launch(kind, &a, s->n, 1, s->st, nullptr, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
@@ -567,6 +574,42 @@ int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
         CHK(pass(s, K_Z_LAST, false, sa, 0, 0));
       else
         CHK(pass(s, K_Z_MID, false, sa, sb, 0));
+    }
+    done += cs;
+  }
+  return GPE_OK;
+}
+
+// The reference's real-time step as separate launches, one per reference kernel
+// and per transform axis (gpe/make_propagator.py:38-101): half-step, FFT z, y, x
+// (cuFFT's fftn), kinetic factor, inverse FFT x, y, z (ifftn, 1/N^3 folded into
+// the kinetic factor), half-step.  9 read + write sweeps of psi per step
+// against 4 for the fused plan; same results to rounding.  Both half-steps of
+// step i use t_mid[i], as the reference does.
+int step_real_unfused(gpe_solve* s, int64_t nsteps, const double* tmid) {
+  const bool dyn = any_dynamic(s);
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t cs = dyn ? std::min<int64_t>(s->chunk, nsteps - done) : nsteps - done;
+    if (dyn) CHK(eval_dynamic(s, tmid + done, (int)cs, false));
+    for (int64_t q = 0; q < cs; ++q) {
+      const int sa = dyn ? (int)q : 0;
+      CHK(pass(s, K_Z_HALF, false, sa, 0, s->pending ? F_PRESCALE : 0));
+      s->pending = false;
+      CHK(pass(s, K_Z_FWD, false, 0, 0, 0));
+      CHK(pass(s, K_Y_FWD, false, 0, 0, 0));
+      CHK(pass(s, K_X_FWD, false, 0, 0, 0));
+      if (s->prec == GPE_C64)
+        launch_kinetic_points<float>(s->psi, s->kin[0], s->kin[1], s->kin[2], s->ka, s->n, s->masked,
+                                     s->kcut * s->kcut, s->st);
+      else
+        launch_kinetic_points<double>(s->psi, s->kin[0], s->kin[1], s->kin[2], s->ka, s->n, s->masked,
+                                      s->kcut * s->kcut, s->st);
+      HIPCHK(hipGetLastError());
+      CHK(pass(s, K_X_INV, false, 0, 0, 0));
+      CHK(pass(s, K_Y_INV, false, 0, 0, 0));
+      CHK(pass(s, K_Z_INV, false, 0, 0, 0));
+      CHK(pass(s, K_Z_HALF, false, sa, 0, 0));
     }
     done += cs;
   }
@@ -1045,6 +1088,17 @@ int gpe_volume_upload(gpe_solve* s, int32_t target, const double* host, int64_t 
 int gpe_step(gpe_solve* s, int64_t nsteps, const double* t_mid) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   return do_step(s, nsteps, t_mid);
+}
+
+int gpe_step_unfused(gpe_solve* s, int64_t nsteps, const double* t_mid) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (s->kind != GPE_REAL) return fail(GPE_ERR_INVALID, "the reference-structured step is real time only");
+  if (nsteps < 0) return fail(GPE_ERR_INVALID, "nsteps must be >= 0");
+  if (nsteps == 0) return GPE_OK;
+  if (any_dynamic(s) && !t_mid)
+    return fail(GPE_ERR_INVALID, "t_mid is required: a time-dependent potential is registered");
+  CHK(set_device(s));
+  return step_real_unfused(s, nsteps, t_mid);
 }
 
 int gpe_step_timed(gpe_solve* s, int64_t nsteps, const double* t_mid, double* ms_per_kind,

@@ -26,6 +26,9 @@ void launch_fill_f64(double* out, long long size, double v, hipStream_t st);
 void launch_add_f64(double* out, const double* in, long long size, hipStream_t st);
 template <typename R> void launch_scale(void* psi, long long count, const double* scal, hipStream_t st);
 template <typename R> void launch_gaussian(void* psi, const double* xa, int n, double sigma, hipStream_t st);
+template <typename R>
+void launch_kinetic_points(void* psi, const void* kx, const void* ky, const void* kz, const double* karr, int n,
+                           int masked, double kcut2, hipStream_t st);
 template <typename R> void launch_from_c128(const void* in, void* out, long long count, hipStream_t st);
 template <typename R> void launch_to_c128(const void* in, void* out, long long count, hipStream_t st);
 template <typename R>

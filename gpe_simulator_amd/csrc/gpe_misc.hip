@@ -153,6 +153,28 @@ __global__ void k_add_f64(double* out, const double* in, long long size) {
 }
 
 // ------------------------------------------------------------ fields --
+// The real-time kinetic factor applied point by point over the whole k-space
+// field, one thread per point, as the reference's elementwise kinetic kernels
+// do (gpe/kernels.py:3-45): the per-axis FP64 factors (1/n^3 folded into x)
+// multiplied as in the fused x pass, the same cutoff rule, one FP64 product
+// rounded once.  Used by the reference-structured step.
+template <typename R>
+__global__ void k_kinetic_points(cpx<R>* psi, const cpx<double>* kx, const cpx<double>* ky, const cpx<double>* kz,
+                                 const double* karr, int n, int masked, double kcut2) {
+  const long long nn = (long long)n * n, count = nn * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = int(e / nn), j = int((e / n) % n), k = int(e % n);
+    cpx<double> f = cmul(kx[i], cmul(ky[j], kz[k]));
+    if (masked) {
+      const double k2 =
+          add_rn(add_rn(mul_rn(karr[i], karr[i]), mul_rn(karr[j], karr[j])), mul_rn(karr[k], karr[k]));
+      if (k2 > kcut2) f = cpx<double>{0.0, 0.0};
+    }
+    psi[e] = cmul_d(psi[e], f);
+  }
+}
+
 template <typename R>
 __global__ void k_scale(cpx<R>* psi, long long count, const double* scal) {
   const double s = scal[1];
@@ -317,6 +339,13 @@ void launch_scale(void* psi, long long count, const double* scal, hipStream_t st
   hipLaunchKernelGGL(k_scale<R>, dim3(nblocks_stride(count, 256)), dim3(256), 0, st, (cpx<R>*)psi, count, scal);
 }
 template <typename R>
+void launch_kinetic_points(void* psi, const void* kx, const void* ky, const void* kz, const double* karr, int n,
+                           int masked, double kcut2, hipStream_t st) {
+  hipLaunchKernelGGL(k_kinetic_points<R>, dim3(nblocks_stride((long long)n * n * n, 256)), dim3(256), 0, st,
+                     (cpx<R>*)psi, (const cpx<double>*)kx, (const cpx<double>*)ky, (const cpx<double>*)kz, karr, n,
+                     masked, kcut2);
+}
+template <typename R>
 void launch_gaussian(void* psi, const double* xa, int n, double sigma, hipStream_t st) {
   hipLaunchKernelGGL(k_gaussian<R>, dim3(nblocks_stride((long long)n * n * n, 256)), dim3(256), 0, st,
                      (cpx<R>*)psi, xa, n, sigma);
@@ -359,6 +388,8 @@ void launch_copy(const void* in, void* out, long long bytes, hipStream_t st) {
   template void launch_cast<R>(const double*, void*, long long, hipStream_t);                                \
   template void launch_scale<R>(void*, long long, const double*, hipStream_t);                               \
   template void launch_gaussian<R>(void*, const double*, int, double, hipStream_t);                          \
+  template void launch_kinetic_points<R>(void*, const void*, const void*, const void*, const double*, int, int,  \
+                                         double, hipStream_t);                                               \
   template void launch_from_c128<R>(const void*, void*, long long, hipStream_t);                             \
   template void launch_to_c128<R>(const void*, void*, long long, hipStream_t);                               \
   template void launch_norm<R>(const void*, long long, double*, unsigned int*, double*, double, int,         \

@@ -49,7 +49,7 @@ static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEve
 static int axis_of(int kind) {
   switch (kind) {
     case K_Y_FWD: case K_Y_INV: case K_Y_LZ: return 1;
-    case K_X_KIN: case K_X_DIAG: case K_X_LZ: return 0;
+    case K_X_KIN: case K_X_DIAG: case K_X_LZ: case K_X_FWD: case K_X_INV: return 0;
     default: return 2;
   }
 }
@@ -85,6 +85,10 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
     case K_X_DIAG: go<0, K_X_DIAG>(n, batch, st, a, t0, t1); break;
     case K_Y_LZ: go<1, K_Y_LZ>(n, batch, st, a, t0, t1); break;
     case K_X_LZ: go<0, K_X_LZ>(n, batch, st, a, t0, t1); break;
+    case K_Z_HALF: go<2, K_Z_HALF>(n, batch, st, a, t0, t1); break;
+    case K_Z_INV: go<2, K_Z_INV>(n, batch, st, a, t0, t1); break;
+    case K_X_FWD: go<0, K_X_FWD>(n, batch, st, a, t0, t1); break;
+    case K_X_INV: go<0, K_X_INV>(n, batch, st, a, t0, t1); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

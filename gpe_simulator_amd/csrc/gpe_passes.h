@@ -38,6 +38,12 @@ enum PassKind : int {
   K_Z_FWD = 8,     // plain FFT_z (psi -> work)
   K_Y_LZ = 9,      // FFT_y, FP64 sum x * k_y * |F|^2 (no write)
   K_X_LZ = 10,     // FFT_x, FP64 sum y * k_x * |F|^2 (no write)
+  // Plain steps of the reference-structured (unfused) real-time step: one
+  // launch per reference kernel and transform axis, for measurement.
+  K_Z_HALF = 11,   // half(slot_a) alone, in real space
+  K_Z_INV = 12,    // plain IFFT_z
+  K_X_FWD = 13,    // plain FFT_x
+  K_X_INV = 14,    // plain IFFT_x
 };
 
 enum PassFlags : int {
@@ -351,7 +357,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // line(s) for the Z passes, so they are read from LDS where used instead of
   // being loaded (and held in registers) ahead of use.
   constexpr bool kStageKin = TF::kFast && KIND == K_X_KIN;
-  constexpr bool kStagePz = TF::kFast && (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST);
+  constexpr bool kStagePz =
+      TF::kFast && (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST || KIND == K_Z_HALF);
   __shared__ cpx<double> kin_lds[kStageKin ? TF::N : 1];
   __shared__ double kx2_lds[kStageKin ? TF::N : 1];   // kx*kx for the momentum cutoff
   __shared__ R pz_lds[kStagePz ? 2 * TF::N : 1];
@@ -423,9 +430,9 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     const int pi = g.pi, pj = g.pj, pk = g.pk;
     bool write = true;
 
-    if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD) {
+    if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD || KIND == K_X_FWD) {
       TF::template fft<-1>(v, lp, t, twp, n);
-    } else if constexpr (KIND == K_Y_INV) {
+    } else if constexpr (KIND == K_Y_INV || KIND == K_Z_INV || KIND == K_X_INV) {
       TF::template fft<+1>(v, lp, t, twp, n);
     } else if constexpr (KIND == K_X_KIN) {
       TF::template fft<-1>(v, lp, t, twp, n);
@@ -462,8 +469,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         }
       }
       TF::template fft<+1>(v, lp, t, twp, n);
-    } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST) {
-      if constexpr (KIND != K_Z_FIRST) {
+    } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST || KIND == K_Z_HALF) {
+      if constexpr (KIND == K_Z_MID || KIND == K_Z_LAST) {
         TF::template fft<+1>(v, lp, t, twp, n);
       } else if (prescale) {
 #pragma unroll
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       } else {
         points(std::false_type{}, std::false_type{});
       }
-      if constexpr (KIND != K_Z_LAST) TF::template fft<-1>(v, lp, t, twp, n);
+      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1>(v, lp, t, twp, n);
     } else if constexpr (KIND == K_Z_DIAG) {
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {

@@ -150,6 +150,13 @@ int gpe_volume_upload(gpe_solve* solve, int32_t target, const double* host, int6
  * Reference: the propagator closures (gpe/make_propagator.py:12-54, 57-103,
  * 109-151) called once per step by launch_stage (gpe/launch.py:8-12). */
 int gpe_step(gpe_solve* solve, int64_t nsteps, const double* t_mid);
+/* The same real-time steps with the reference's launch structure: one
+ * launch per reference kernel and transform axis (half-step, FFT z, y, x,
+ * kinetic factor, inverse FFT x, y, z, half-step; gpe/make_propagator.py:38-101
+ * with cuFFT's fftn / ifftn as three axis passes each), 9 read + write sweeps
+ * of psi per step instead of the fused plan's 4.  Same results to rounding.
+ * For measuring what the fusion gains; real-time solves only. */
+int gpe_step_unfused(gpe_solve* solve, int64_t nsteps, const double* t_mid);
 /* Norm per step of the last imaginary-time gpe_step call (before
  * renormalisation), `count` values ending with the last step. */
 int gpe_norm_history(gpe_solve* solve, double* out, int64_t count);

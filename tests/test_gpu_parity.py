@@ -293,3 +293,44 @@ def test_scalar_only_diagnostics_keep_the_last_spectrum(require_gpu):
     np.testing.assert_array_equal(occ, again["view_occupation"])
     assert not np.array_equal(first["view_occupation"], again["view_occupation"])
     solve.close()
+
+
+@pytest.mark.parametrize("n", [18, 64, 96, 256])
+@pytest.mark.parametrize("precision,tol_fused,tol_oracle", [("c128", 1e-12, 1e-11), ("c64", 2e-5, 3e-5)])
+def test_reference_structured_step_matches_fused_and_oracle(n, precision, tol_fused, tol_oracle, require_gpu):
+    """gpe_step_unfused (one launch per reference kernel and axis) against the
+    fused plan and the oracle: trap, moving beam (per-step time slots),
+    momentum cutoff and an absorber volume (the complex static potential)."""
+    from gpe_simulator_amd.engine import midpoint_times
+    from gpe_simulator_amd import _lib
+    length, steps, dt = 14.0, 6, 1e-3
+    static, beam = "x**2 + y**2 + z**2", "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)"
+    kcut = 0.6 * np.pi * n / length
+    problem, unfused = make(n, length, "real", dt, 40.0, static, beam, kcut, precision=precision)
+    _, fused = make(n, length, "real", dt, 40.0, static, beam, kcut, precision=precision)
+    x = problem.x
+    absorber = 3.0 * np.maximum(np.abs(x) - 0.3 * length, 0)
+    volume = -(absorber[:, None, None] + absorber[None, :, None] + absorber[None, None, :])
+    for solve in (unfused, fused):
+        solve.volume_upload(_lib.TGT_VOL_IM, volume.ravel())
+    problem.v_static = problem.v_static + 1j * volume
+    psi0 = random_state(n, length, seed=9)
+    tm = midpoint_times(0, steps, dt)
+    for solve in (unfused, fused):
+        solve.upload(psi0)
+    unfused.step_unfused(steps, tm)
+    fused.step(steps, tm)
+    got, ref = unfused.to_numpy(), fused.to_numpy()
+    unfused.close()
+    fused.close()
+    assert rel_l2(got, ref) < tol_fused
+    expected = problem.run(psi0, steps)
+    assert rel_l2(got, expected) < tol_oracle
+
+
+def test_reference_structured_step_is_real_time_only(require_gpu):
+    from gpe_simulator_amd import _lib
+    _, solve = make(16, 8.0, "imaginary", 1e-3, 10.0)
+    with pytest.raises(_lib.GpeInvalid, match="real time only"):
+        solve.step_unfused(2)
+    solve.close()
