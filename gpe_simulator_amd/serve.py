@@ -96,6 +96,21 @@ class Server:
         self.grid_point_steps = 0
         self.max_active = 0
 
+    def warm(self, shapes):
+        """Tune the pass variants of every (side, precision, kind) in `shapes`
+        on the server's context before requests arrive, so that no request
+        waits for tuning (make_solve tunes a shape the first time it sees it).
+        precision: "c64" / "c128"; kind: "imaginary" / "real".  Complex64
+        ground states also need the complex128 shape of their polish.
+        Returns the seconds spent."""
+        from . import tune
+        from .engine import precision_code
+        t0 = time.perf_counter()
+        for n, precision, kind in shapes:
+            tune.ensure_tuned(int(n), precision_code(precision), _lib.IMAG if kind == "imaginary" else _lib.REAL,
+                              ctx=self.ctx)
+        return time.perf_counter() - t0
+
     # -- solve lifecycle
     def _new_solve(self, req, precision):
         x, k = get_grids(req.n, req.length)

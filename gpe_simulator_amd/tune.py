@@ -4,11 +4,21 @@ Every pass kind of a step (z_first, z_mid, z_last, y_fwd, y_inv, x_kin) reads
 and writes the field in the same canonical layout, so each kind can run any of
 the compiled tile variants of its (side, precision): threads per pencil,
 pencils per block and the waves per SIMD its register budget allows
-(gpe_shapes.h).  The tuner times each variant on scratch solves of the real
-shape (alone, or as a batch of `batch` members, since small grids behave
-differently when many solves share a launch), with HIP events around every
-pass, and keeps the fastest variant per kind.  The choice becomes the
-context's default for new solves of that (side, precision, kind).
+(gpe_shapes.h).  The tuner works on scratch solves of the real shape (alone,
+or as a batch of `batch` members, since small grids behave differently when
+many solves share a launch) in two stages:
+
+1. every variant is timed with HIP events around every pass, giving the
+   fastest variant per kind;
+2. that choice and the uniform choices (one variant for every kind) are timed
+   as whole steps, and the fastest is refined by coordinate descent, one kind
+   at a time, on whole-step time.
+
+The second stage is what decides.  The per-pass events cost a few µs per
+launch, which is most of a pass at 64^3-128^3, and a pass's speed depends on
+what ran before it.  There the first stage alone chose up to 10% slower steps
+than the descent.  The choice becomes the context's default for new solves
+of that (side, precision, kind).
 
 Results are cached on the context; variants a caller set on the context
 explicitly are never replaced.  Set the environment variable
@@ -19,12 +29,17 @@ from __future__ import annotations
 
 import os
 import threading
+import time
 from dataclasses import dataclass, field
 
 from . import _lib
 from .grids import get_grids
 
 STEP_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin")
+# pass kinds of a step in the order the descent visits them (most time first)
+STEP_SEQUENCE = {_lib.REAL: ("z_mid", "x_kin", "y_fwd", "y_inv", "z_first", "z_last"),
+                 _lib.IMAG: ("z_first", "z_last", "x_kin", "y_fwd", "y_inv")}
+GAIN = 0.005        # a descent move must shorten the step by more than this fraction
 _lock = threading.Lock()
 
 
@@ -37,6 +52,9 @@ class Tuning:
     variants: list                       # [(tp, pencils per block, waves)]
     choice: list                         # variant per pass kind (PASS_KINDS order)
     timings: dict = field(default_factory=dict)   # variant -> {kind: avg us}
+    event_choice: list = field(default_factory=list)   # fastest variant per kind by pass events
+    step_us: float = float("nan")        # whole-step time of `choice`
+    event_step_us: float = float("nan")  # whole-step time of `event_choice`
 
     def report(self):
         lines = [f"n={self.n} precision={'c64' if self.precision == _lib.C64 else 'c128'} "
@@ -45,7 +63,8 @@ class Tuning:
             t = self.timings.get(v, {})
             lines.append(f"  v{v} tp={geo[0]} b={geo[1]} waves={geo[2]}: " +
                          ", ".join(f"{k} {t[k]:.2f}" for k in STEP_KINDS if k in t))
-        lines.append("  choice: " + ", ".join(f"{k}=v{self.choice[_lib.PASS_KINDS.index(k)]}" for k in STEP_KINDS))
+        lines.append("  choice: " + ", ".join(f"{k}=v{self.choice[_lib.PASS_KINDS.index(k)]}" for k in STEP_KINDS)
+                     + f" ({self.step_us:.2f} us/step; per-pass choice {self.event_step_us:.2f})")
         return "\n".join(lines)
 
 
@@ -97,18 +116,88 @@ def measure(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
     return variants, out
 
 
+class StepClock:
+    """Whole-step time of scratch solves under a given choice of variants:
+    device time between two marks on a solo solve's stream, or host time
+    around synchronous graph replays of a batch."""
+
+    def __init__(self, n, precision, kind, batch, ctx, steps=None, rounds=3):
+        self.solves = [_scratch(n, precision, kind, ctx) for _ in range(batch)]
+        self.group = _lib.Batch(self.solves, ctx=ctx) if batch > 1 else None
+        # about 2e8 grid-point-steps (a few ms) per timed round
+        self.steps = steps or max(8, min(100, int(2e8 / (n ** 3 * batch))))
+        self.rounds = rounds
+
+    def __call__(self, choice):
+        for s in self.solves:
+            s.set_variants(choice)
+        best = float("inf")
+        if self.group is None:
+            s = self.solves[0]
+            s.step(2)
+            for _ in range(self.rounds):
+                s.mark(0)
+                s.step(self.steps)
+                s.mark(1)
+                best = min(best, s.marked_ms() * 1e3 / self.steps)
+        else:
+            self.group.step(2, graph=True)          # captures the graph of this choice
+            for _ in range(self.rounds):
+                t0 = time.perf_counter()
+                self.group.step(self.steps, graph=True)
+                best = min(best, (time.perf_counter() - t0) * 1e6 / self.steps)
+        return best
+
+    def close(self):
+        if self.group is not None:
+            self.group.close()
+        for s in self.solves:
+            s.close()
+
+
+def refine(clock, start, kind, nvariants):
+    """Fastest whole step from `start` and the uniform choices, improved by
+    coordinate descent over the step's pass kinds (two sweeps, moves that
+    gain more than GAIN).  Returns (choice, us per step)."""
+    candidates = [list(start)] + [[v] * len(_lib.PASS_KINDS) for v in range(nvariants)]
+    timed = [(clock(c), c) for c in candidates]
+    best_us, best = min(timed, key=lambda item: item[0])
+    for _ in range(2):
+        moved = False
+        for name in STEP_SEQUENCE[kind]:
+            idx = _lib.PASS_KINDS.index(name)
+            for v in range(nvariants):
+                if v == best[idx]:
+                    continue
+                trial = list(best)
+                trial[idx] = v
+                us = clock(trial)
+                if us < best_us * (1 - GAIN):
+                    best, best_us, moved = trial, us, True
+        if not moved:
+            break
+    return best, best_us
+
+
 def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
-    """Time every variant and return the fastest per pass kind (a Tuning)."""
+    """Time every variant per pass, then decide on whole-step time (a Tuning)."""
     precision = int(precision)
     kind = int(kind)
+    ctx = ctx or _lib.default_context()
     variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx)
-    choice = [0] * len(_lib.PASS_KINDS)
+    event_choice = [0] * len(_lib.PASS_KINDS)
     for name in STEP_KINDS:
         measured = [(timings[v][name], v) for v in timings if name in timings[v]]
         if measured:
-            choice[_lib.PASS_KINDS.index(name)] = min(measured)[1]
+            event_choice[_lib.PASS_KINDS.index(name)] = min(measured)[1]
+    clock = StepClock(n, precision, kind, batch, ctx)
+    try:
+        event_us = clock(event_choice)
+        choice, step_us = refine(clock, event_choice, kind, len(variants))
+    finally:
+        clock.close()
     return Tuning(n=n, precision=precision, kind=kind, batch=batch, variants=variants, choice=choice,
-                  timings=timings)
+                  timings=timings, event_choice=event_choice, step_us=step_us, event_step_us=event_us)
 
 
 def ensure_tuned(n, precision, kind, ctx=None, batch=1):

@@ -172,3 +172,22 @@ def test_sweep_over_mixed_sides(require_gpu):
     for req, res in zip(reqs, sweep_batched(reqs)):
         assert res.converged
         assert res.mu == pytest.approx(solve_sequential(req).mu, rel=1e-8)
+
+
+def test_server_warm_up_tunes_every_shape(require_gpu):
+    """Server.warm tunes each (side, precision, kind) once on its context;
+    later solves of those shapes start from the tuned variants."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.serve import Server
+    ctx = _lib.Context(0)
+    server = Server(ctx=ctx)
+    shapes = [(64, "c64", "imaginary"), (64, "c128", "imaginary"), (96, "c64", "real")]
+    seconds = server.warm(shapes)
+    assert seconds > 0
+    for n, prec, kind in shapes:
+        key = (n, _lib.C64 if prec == "c64" else _lib.C128, _lib.IMAG if kind == "imaginary" else _lib.REAL, 1)
+        assert ctx.tunings[key] is not None
+        assert ctx.variant_defaults[key[:3]] == ctx.tunings[key].choice
+    again = server.warm(shapes)
+    assert again < seconds
+    ctx.close()

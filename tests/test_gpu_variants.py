@@ -84,6 +84,10 @@ def test_autotune_sets_context_defaults(require_gpu):
     count = variant_count(96, "c64")
     assert all(0 <= v < count for v in result.choice)
     assert set(result.timings) == set(range(count))
+    assert all(0 <= v < count for v in result.event_choice)
+    # the whole-step stage starts from the per-pass choice and keeps the faster
+    # step (5% covers the noise between the two separate timings)
+    assert 0 < result.step_us <= result.event_step_us * 1.05
     ctx.set_variants(96, _lib.C64, _lib.IMAG, result.choice)
     x, k = orc.get_grids(96, 20.0)
     with_ctx = make_solve(96, x, k, "imaginary", 5e-4, 100.0, static="x**2 + y**2 + z**2", precision="c64",
