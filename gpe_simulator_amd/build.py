@@ -47,7 +47,7 @@ def variants():
     text = (CSRC / "gpe_shapes.h").read_text()
     block = text[text.index("#define GPE_VARIANTS(X)"):text.index("namespace gpe")]
     found = [tuple(int(v) for v in m) for m in
-             re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*\d+,\s*\d+,\s*\d+\)", block)]
+             re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*\d+,\s*\d+,\s*\d+,\s*\d+\)", block)]
     return [(0, 0, 0), (1, 0, 0)] + [f[:3] for f in found]
 
 

@@ -73,7 +73,7 @@ struct PassVariant {
 const PassVariant kVariantTable[] = {
     {0, 0, 0, 64, 1, 1, launch_pass_p0_n0_v0, pass_blocks_p0_n0_v0},
     {1, 0, 0, 64, 1, 1, launch_pass_p1_n0_v0, pass_blocks_p1_n0_v0},
-#define GPE_VARIANT_ENTRY(P, N, V, TP, B, W) \
+#define GPE_VARIANT_ENTRY(P, N, V, TP, B, W, RD) \
   {P, N, V, TP, B, W, launch_pass_p##P##_n##N##_v##V, pass_blocks_p##P##_n##N##_v##V},
     GPE_VARIANTS(GPE_VARIANT_ENTRY)
 #undef GPE_VARIANT_ENTRY

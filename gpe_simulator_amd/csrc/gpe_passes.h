@@ -98,7 +98,7 @@ struct PassArgs {
 };
 
 // ------------------------------------------------------------ transforms --
-template <typename R, int N_, int TP_, int B_, int W_>
+template <typename R, int N_, int TP_, int B_, int W_, int RD_ = 1>
 struct FastTF {
   static constexpr bool kFast = true;
   static constexpr int N = N_, TP = TP_, B = B_;
@@ -108,6 +108,12 @@ struct FastTF {
   // Waves per SIMD the register allocation must allow (a tuning parameter:
   // more waves hide more latency but cap the registers per thread).
   static constexpr int MIN_WAVES = W_;
+  // Exchange rounds of the Z passes: with 2, half of the block's pencils go
+  // through the LDS exchange at a time, which halves its LDS (more blocks
+  // per CU) at the cost of a second pair of barriers.  A Z-pass wave's
+  // pencils are consecutive, so each wave belongs to one half.
+  static constexpr int ROUNDS = RD_;
+  static_assert(ROUNDS == 1 || (ROUNDS == 2 && (B / 2) % (64 / TP) == 0), "a round must hold whole waves");
   static constexpr int LSTRIDE = TP + 1;           // padded LDS row
   // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
   // pencils, so an even stride would put them on the same LDS bank.
@@ -131,8 +137,11 @@ struct FastTF {
     __syncthreads();
   }
 
-  template <int DIR>
-  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const TwT* twl, int) {
+  // RNDS exchange rounds; `group` is the round in which this thread's
+  // pencil uses the LDS (0 with one round).  Every thread passes every
+  // barrier; only the LDS accesses depend on the group.
+  template <int DIR, int RNDS = 1>
+  __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const TwT* twl, int, int group = 0) {
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
@@ -143,22 +152,30 @@ struct FastTF {
       // are read from LDS as they are used, not all up front
       if constexpr (k1 % 4 == 0) __builtin_amdgcn_sched_barrier(0);
     });
-    __syncthreads();
-    static_for<0, M>([&](auto k1c) {
-      constexpr int k1 = decltype(k1c)::value;
-      lp[k1 * LSTRIDE + t] = v[k1];
-    });
-    __syncthreads();
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const int k1 = t + TP * q;
-      cpx<R> u[TP];
-      static_for<0, TP>([&](auto sc) { u[decltype(sc)::value] = lp[k1 * LSTRIDE + decltype(sc)::value]; });
-      dft<TP, DIR, R>(u);
-      static_for<0, TP>([&](auto k2c) {
-        constexpr int k2 = decltype(k2c)::value;
-        v[q + Q * k2] = u[k2];
-      });
+    static_for<0, RNDS>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      const bool mine = RNDS == 1 || group == r;
+      __syncthreads();
+      if (mine) {
+        static_for<0, M>([&](auto k1c) {
+          constexpr int k1 = decltype(k1c)::value;
+          lp[k1 * LSTRIDE + t] = v[k1];
+        });
+      }
+      __syncthreads();
+      if (mine) {
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          const int k1 = t + TP * q;
+          cpx<R> u[TP];
+          static_for<0, TP>([&](auto sc) { u[decltype(sc)::value] = lp[k1 * LSTRIDE + decltype(sc)::value]; });
+          dft<TP, DIR, R>(u);
+          static_for<0, TP>([&](auto k2c) {
+            constexpr int k2 = decltype(k2c)::value;
+            v[q + Q * k2] = u[k2];
+          });
+        });
+      }
     });
   }
 };
@@ -169,6 +186,7 @@ struct GenTF {
   static constexpr int N = 0, TP = 64, B = 1, M = 16;
   static constexpr int THREADS = 64;
   static constexpr int MIN_WAVES = 1;
+  static constexpr int ROUNDS = 1;
   static constexpr int PS = 1024;
   static constexpr int LDS_ELEMS = 1024;
 
@@ -178,9 +196,9 @@ struct GenTF {
   static constexpr int TWL_ELEMS = 1;
   __device__ static __forceinline__ void stage_twiddles(TwT*, const cpx<double>*) {}
 
-  template <int DIR>
+  template <int DIR, int RNDS = 1>
   __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* __restrict__ tw,
-                                             int n) {
+                                             int n, int = 0) {
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < M; ++m)
@@ -335,7 +353,10 @@ __device__ __forceinline__ void load_tile(cpx<R>* v, const cpx<R>* src, long lon
 
 template <class TF, int AXIS, int KIND, typename R>
 __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R> a) {
-  __shared__ cpx<R> lds[TF::LDS_ELEMS];
+  // Only Z passes exchange in rounds: there a wave's lanes own whole
+  // consecutive pencils; in the X/Y passes a pencil's threads span waves.
+  constexpr int kRounds = AXIS == 2 ? TF::ROUNDS : 1;
+  __shared__ cpx<R> lds[TF::LDS_ELEMS / kRounds];
   __shared__ typename TF::TwT twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
@@ -351,7 +372,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   }
   const long long stride = AXIS == 2 ? 1LL : (AXIS == 1 ? (long long)n : (long long)n * n);
   const int ntiles = tile_count<TF, AXIS>(n);
-  cpx<R>* lp = lds + b * TF::PS;
+  const int group = b / (TF::B / kRounds);
+  cpx<R>* lp = lds + (b % (TF::B / kRounds)) * TF::PS;
   // Block-shared tables staged next to the twiddles (the twiddle barrier
   // covers them): the FP64 x kinetic factors for X_KIN and the z potential
   // line(s) for the Z passes, so they are read from LDS where used instead of
@@ -431,11 +453,11 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     bool write = true;
 
     if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD || KIND == K_X_FWD) {
-      TF::template fft<-1>(v, lp, t, twp, n);
+      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_INV || KIND == K_Z_INV || KIND == K_X_INV) {
-      TF::template fft<+1>(v, lp, t, twp, n);
+      TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_X_KIN) {
-      TF::template fft<-1>(v, lp, t, twp, n);
+      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
       if (s.imag) {
         const double eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
 #pragma unroll
@@ -468,10 +490,10 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      TF::template fft<+1>(v, lp, t, twp, n);
+      TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST || KIND == K_Z_HALF) {
       if constexpr (KIND == K_Z_MID || KIND == K_Z_LAST) {
-        TF::template fft<+1>(v, lp, t, twp, n);
+        TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
       } else if (prescale) {
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
@@ -519,7 +541,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       } else {
         points(std::false_type{}, std::false_type{});
       }
-      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1>(v, lp, t, twp, n);
+      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Z_DIAG) {
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {
@@ -541,7 +563,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           acc[2] += a2 * a2;
         }
       }
-      TF::template fft<-1>(v, lp, t, twp, n);
+      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
       // Angular momentum by Parseval along one axis:
       //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
@@ -549,7 +571,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
-      TF::template fft<-1>(v, lp, t, twp, n);
+      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
       const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);
       double part = 0.0;
 #pragma unroll
@@ -562,7 +584,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       acc[0] += coord * part;
       write = false;
     } else if constexpr (KIND == K_X_DIAG) {
-      TF::template fft<-1>(v, lp, t, twp, n);
+      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
       const double ky = gload(s.karr + pj), kz = gload(s.karr + pk);
       const int fq = freq_index(pj, n), fr = freq_index(pk, n);
       const float view_scale = s.nview ? float(s.nview) / (1.7320508f * float(s.vh)) : 0.f;

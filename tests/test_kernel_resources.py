@@ -50,7 +50,8 @@ def current_variants():
     text = (ROOT / "gpe_simulator_amd" / "csrc" / "gpe_shapes.h").read_text()
     block = text[text.index("#define GPE_VARIANTS(X)"):text.index("namespace gpe")]
     out = {(0, 0, 0): None, (1, 0, 0): None}
-    for p, n, v, _tp, _b, w in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", block):
+    for p, n, v, _tp, _b, w, _rd in re.findall(
+            r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", block):
         out[(int(p), int(n), int(v))] = int(w)
     return out
 
