@@ -4,10 +4,13 @@ counterpart of gpe/benchmark.py of the reference.
 CPU: the generated config holds the reference's ten cases with its settings
 (gpe/benchmark.py:18-55) and passes the schema-2 validator; the report files
 have the reference's row keys, order and completeness flag (:112-179); the
-command line rejects what the reference rejects (:188-199).
-GPU: a full run at 32^3 writes a complete report whose norms obey the physics
-of each case (imaginary time renormalises; real time without absorber or
-cutoff conserves the norm; the cutoff only removes weight).
+command line rejects what the reference rejects (:188-199); and the
+reference's own harness tests (tests/test_benchmark.py:36-102) restated with
+fakes: the timer's order, repeats that restart from a fresh Gaussian with the
+same per-step figures, and a failed setup that still releases the stage.
+GPU: a full run at 32^3 and 48^3 writes a complete report whose norms obey
+the physics of each case (imaginary time renormalises; real time without
+absorber or cutoff conserves the norm; the cutoff only removes weight).
 """
 
 import csv
@@ -170,3 +173,104 @@ def test_full_run(tmp_path, precision, norm_tol, require_gpu):
     for n in (32, 48):
         assert (folder / f"config_N{n}.yaml").exists()
     assert "pass tile variants" in (folder / "engine_config.txt").read_text()
+
+
+class FakeSolve:
+    """Records the calls the harness makes on a solve."""
+
+    def __init__(self, calls):
+        self.calls = calls
+
+    def sync(self):
+        self.calls.append("drain stream")
+
+    def mark(self, which):
+        self.calls.append("start mark" if which == 0 else "stop mark")
+
+    def marked_ms(self):
+        self.calls.append("wait for stop mark")
+        return 12.5
+
+
+def test_timer_brackets_the_batch(monkeypatch):
+    """gpe/benchmark.py's measure order (tests/test_benchmark.py:36-54 of the
+    reference): drain, wall clock, start, work, stop, wait, wall clock."""
+    calls = []
+    times = iter((1.0, 1.02))
+
+    def clock():
+        calls.append("wall clock")
+        return next(times)
+
+    monkeypatch.setattr(benchmark, "perf_counter", clock)
+    wall, device = benchmark.timed(FakeSolve(calls), lambda: calls.append("work"))
+    assert wall == pytest.approx(0.02) and device == 12.5
+    assert calls == ["drain stream", "wall clock", "start mark", "work", "stop mark", "wait for stop mark",
+                     "wall clock"]
+
+
+def test_repeats_reset_state_and_report_per_step_times(monkeypatch):
+    """Each timed repeat starts from a freshly built Gaussian; the row holds
+    per-step medians and spreads (tests/test_benchmark.py:56-92 of the
+    reference, same figures)."""
+    import sys
+    from types import SimpleNamespace
+    from unittest.mock import Mock
+    calls = []
+    stage = SimpleNamespace(name="imaginary_none", N_grids=2, initial_state={"type": "gaussian", "sigma": 10},
+                            psi=None, solve=None, steps=0)
+    stage.prepare = lambda: setattr(stage, "solve", FakeSolve([]))
+    stage.release = Mock()
+
+    def launch(item):
+        calls.append(item.steps)
+        if item.steps != 2:                       # the first step and every timed repeat
+            assert item.psi == "fresh gaussian", item.psi
+        item.psi = "advanced"
+        return item.psi
+
+    def gaussian(solve, sigma):
+        assert solve is stage.solve and sigma == 10
+        return "fresh gaussian"
+
+    stage.psi = "fresh gaussian"                  # prepare builds the initial state
+    timings = iter(((0.01, 9), (0.008, 7), (0.012, 11), (0.016, 15)))
+    monkeypatch.setattr(benchmark, "timed", lambda solve, fn: (fn(), next(timings))[1])
+    clock = iter((0.0, 0.02))
+    monkeypatch.setattr(benchmark, "perf_counter", lambda: next(clock))
+    monkeypatch.setitem(sys.modules, "gpe_simulator_amd.launch", SimpleNamespace(launch_stage=launch))
+    monkeypatch.setitem(sys.modules, "gpe_simulator_amd.initial_states", SimpleNamespace(gaussian=gaussian))
+    monkeypatch.setitem(sys.modules, "gpe_simulator_amd.helper", SimpleNamespace(get_N=lambda psi: 1.0))
+    row = benchmark.benchmark_stage(stage, steps=4, warmup=2, repeats=3)
+    assert calls == [1, 2, 4, 4, 4]
+    stage.release.assert_called_once()
+    assert row["setup_ms"] == pytest.approx(20)
+    assert row["first_step_ms"] == pytest.approx(10)
+    assert row["wall_ms_per_step"] == pytest.approx(3)
+    assert row["wall_min_ms_per_step"] == pytest.approx(2)
+    assert row["wall_max_ms_per_step"] == pytest.approx(4)
+    assert row["cuda_ms_per_step"] == pytest.approx(2.75)
+    assert row["steps_per_second"] == pytest.approx(1000 / 3)
+    assert len(row["samples"]) == 3 and all(s["final_norm"] == 1.0 for s in row["samples"])
+
+
+def test_failed_setup_releases_stage():
+    """tests/test_benchmark.py:94-102 of the reference."""
+    from types import SimpleNamespace
+    from unittest.mock import Mock
+    stage = SimpleNamespace(prepare=Mock(side_effect=RuntimeError("allocation failed")), release=Mock())
+    with pytest.raises(RuntimeError, match="allocation failed"):
+        benchmark.benchmark_stage(stage, 4, 2, 3)
+    stage.release.assert_called_once()
+
+
+@pytest.mark.parametrize("norm,message", [(float("nan"), "non-finite wavefunction after repeat 2"),
+                                          (float("inf"), "non-finite wavefunction after repeat 2"),
+                                          (0.0, "invalid norm 0.0")])
+def test_invalid_state_after_a_repeat_is_an_error(monkeypatch, norm, message):
+    import sys
+    from types import SimpleNamespace
+    stage = SimpleNamespace(name="real_none", psi="psi")
+    monkeypatch.setitem(sys.modules, "gpe_simulator_amd.helper", SimpleNamespace(get_N=lambda psi: norm))
+    with pytest.raises(RuntimeError, match=message):
+        benchmark.check_state(stage, 2)
