@@ -177,3 +177,33 @@ def test_cli_check_does_not_load_the_engine(tmp_path):
     assert out.returncode == 0, out.stderr
     assert "Valid configuration: 1 stages" in out.stdout
     assert "NOENGINE" in out.stdout
+
+
+@pytest.mark.parametrize("i,interval,expected", [(0, 1, False), (1, 0, False), (2, None, False),
+                                                 (1, 2, False), (2, 2, True), (6, 3, True)])
+def test_output_schedule(i, interval, expected):
+    """The output schedule predicate (tests/test_diagnostics.py:186-189 of the
+    reference): never at step 0, never for a 0 or missing interval."""
+    from gpe_simulator_amd.helper import is_due
+    assert is_due(i, interval) == expected
+
+
+def test_next_due_stops_at_every_due_output_and_the_last_step():
+    """launch_stage hands the engine every step up to the next step at which
+    save_outputs has work, so outputs land on the same steps as with one call
+    per step."""
+    from types import SimpleNamespace
+    from gpe_simulator_amd.helper import is_due
+    from gpe_simulator_amd.launch import MAX_STEPS_PER_CALL, next_due
+    stage = SimpleNamespace(steps=50, save_params={"psi_interval": 7, "scalar_interval": 0, "nk_interval": 5},
+                            monitor_params={"print_scalar_interval": 0, "show_nk_interval": 12})
+    stops, i = [], 0
+    while i < stage.steps:
+        i = next_due(stage, i)
+        stops.append(i)
+    due = [s for s in range(1, 51) if any(is_due(s, iv) for iv in (7, 5, 12))] + [50]
+    assert stops == sorted(set(due))
+    long = SimpleNamespace(steps=3 * MAX_STEPS_PER_CALL, save_params={"psi_interval": 0, "scalar_interval": 0,
+                                                                      "nk_interval": 0},
+                           monitor_params={"print_scalar_interval": 0})
+    assert next_due(long, 0) == MAX_STEPS_PER_CALL
