@@ -207,3 +207,58 @@ def test_next_due_stops_at_every_due_output_and_the_last_step():
                                                                       "nk_interval": 0},
                            monitor_params={"print_scalar_interval": 0})
     assert next_due(long, 0) == MAX_STEPS_PER_CALL
+
+
+def host_stage(tmp_path, n=4):
+    """A parsed stage holding a host psi (no solve: the snapshot code then
+    reads and writes host arrays only)."""
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, _ = write_config(tmp_path)
+    item = parse_yaml(path)[0]
+    if n != item.N_grids:
+        raise ValueError("host_stage builds the 4^3 config")
+    rng = np.random.default_rng(2)
+    item.psi = rng.normal(size=(n,) * 3) + 1j * rng.normal(size=(n,) * 3)
+    return item
+
+
+def test_snapshot_roundtrip_preserves_values_and_checks_metadata(tmp_path):
+    """tests/test_workflow.py:110-130 of the reference."""
+    from gpe_simulator_amd.state_io import load_wavefunction, save_wavefunction
+    item = host_stage(tmp_path)
+    path = item.data_dir / "psi_00000006.npy"
+    save_wavefunction(path, item, 6)
+    loaded = load_wavefunction(path, item)
+    np.testing.assert_array_equal(loaded, item.psi)
+    metadata = json.loads(path.with_suffix(".json").read_text())
+    assert (metadata["step"], metadata["time"]) == (6, 0.75)
+    assert metadata["stage"] == item.config
+    item.simulation_range += 1
+    with pytest.raises(ValueError, match="simulation range"):
+        load_wavefunction(path, item)
+    item.simulation_range -= 1
+    item.N_atoms += 1
+    with pytest.raises(ValueError, match="reference N_atom"):
+        load_wavefunction(path, item)
+    item.N_atoms -= 1
+    bad = tmp_path / "bad.npy"
+    np.save(bad, np.ones((4, 4, 4), dtype=np.complex64))
+    with pytest.raises(ValueError, match="complex128"):
+        load_wavefunction(bad, item)
+    np.save(bad, np.ones((5, 5, 5), dtype=np.complex128))
+    with pytest.raises(ValueError, match="N_grids"):
+        load_wavefunction(bad, item)
+
+
+def test_snapshot_load_checks_legacy_grid_companion(tmp_path):
+    """tests/test_workflow.py:132-141 of the reference."""
+    from gpe_simulator_amd.state_io import load_wavefunction
+    item = host_stage(tmp_path)
+    path = tmp_path / "psi_final.npy"
+    np.save(path, item.psi)
+    grid = path.with_name("psi_final_grid.npz")
+    np.savez(grid, x_array=item.x_array + 0.25, k_array=item.k_array)
+    with pytest.raises(ValueError, match="grid does not match"):
+        load_wavefunction(path, item)
+    np.savez(grid, x_array=item.x_array, k_array=item.k_array)
+    np.testing.assert_array_equal(load_wavefunction(path, item), item.psi)
