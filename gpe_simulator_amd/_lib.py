@@ -34,7 +34,7 @@ EXPORTS = (
     "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
-    "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused",
+    "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
 )
 
 
@@ -93,6 +93,7 @@ def _declare(lib):
         "gpe_mark": (c_int, [vp, i32]),
         "gpe_mark_elapsed": (c_int, [vp, dp]),
         "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
+        "gpe_step_stamped": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
         "gpe_angular_momentum": (c_int, [vp, dp]),
         "gpe_batch_create": (c_int, [vp, POINTER(vp), i32, POINTER(vp)]),
@@ -338,6 +339,16 @@ class Solve:
         check(lib().gpe_step_timed(self.handle, int(nsteps), dptr(tm) if tm is not None else None, dptr(ms),
                                    i64ptr(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def step_stamped(self, nsteps, t_mid=None):
+        """gpe_step with device wall-clock spans per launch:
+        {kind: (summed us, launches)}."""
+        tm = None if t_mid is None else np.ascontiguousarray(t_mid, dtype=np.float64)
+        us = np.zeros(len(PASS_KINDS))
+        cnt = np.zeros(len(PASS_KINDS), dtype=np.int64)
+        check(lib().gpe_step_stamped(self.handle, int(nsteps), dptr(tm) if tm is not None else None, dptr(us),
+                                     i64ptr(cnt)))
+        return {k: (float(us[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
 
     def set_variants(self, variants):
         """Pass variant per kind (sequence in PASS_KINDS order, or a dict)."""

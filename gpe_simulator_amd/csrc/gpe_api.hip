@@ -53,7 +53,9 @@ struct PassArgsRaw {
   int slot_a, slot_b;
   int flags;
   int hist_index;
+  int stamp_index;
 };
+constexpr long long kMaxStamps = 1LL << 22;  // wall-clock slots of one stamped call (32 MiB)
 static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<float>), "pass argument layout");
 static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<double>), "pass argument layout");
 
@@ -243,6 +245,16 @@ struct gpe_solve {
   bool timing = false;
   PassTimer timer;
   hipEvent_t mark[2] = {nullptr, nullptr};  // gpe_mark start / stop
+  unsigned long long* stamps = nullptr;   // kMaxStamps wall-clock slots
+  bool stamping = false;
+  bool stamp_chained = false;             // the stream's last command was a stamped pass
+  long long stamp_used = 0, stamp_last = 0, stamp_last_count = 0;
+  struct StampSpan {
+    int kind;
+    long long begin, begin_count;         // slots whose latest value starts the span
+    long long end, end_count;             // the launch's per-block end slots
+  };
+  std::vector<StampSpan> stamp_spans;
 };
 
 namespace {
@@ -315,6 +327,7 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
   d.red = s->red;
   d.scal = s->scal;
   d.hist = s->hist;
+  d.stamps = s->stamps;
   d.nkocc = s->nkocc;
   d.nkcnt = s->nkcnt;
   d.dx3 = s->dx3;
@@ -355,7 +368,27 @@ int ensure_red(gpe_solve* s, long long need) {
 }
 
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
-  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index};
+  int stamp_index = 0;
+  const long long nblk = s->blocks(kind, s->n);
+  if (s->stamping && kind < kPassKinds && s->stamp_used + 1 + nblk <= kMaxStamps) {
+    // the span begins at the previous timed launch's block ends, or at a
+    // reference time when other work ran since
+    long long begin = s->stamp_last, begin_count = s->stamp_last_count;
+    if (!s->stamp_chained) {
+      begin = s->stamp_used++;
+      begin_count = 1;
+      launch_stamp_now(s->stamps + begin, s->st);
+    }
+    stamp_index = (int)s->stamp_used;
+    s->stamp_used += nblk;
+    s->stamp_spans.push_back({kind, begin, begin_count, stamp_index, nblk});
+    s->stamp_last = stamp_index;
+    s->stamp_last_count = nblk;
+    // the norm finisher follows an imaginary Z_LAST on the stream
+    s->stamp_chained = !(kind == K_Z_LAST && (flags & F_NORM));
+    flags |= F_STAMP;
+  }
+  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index};
   hipEvent_t stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &stop));
   hipError_t e = s->launch(kind, &a, s->n, 1, s->st, nullptr, stop);
@@ -493,6 +526,7 @@ bool any_dynamic(gpe_solve* s) { return s->dyn[0].active || s->dyn[1].active || 
 int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream_t stream = nullptr) {
   hipStream_t st = stream ? stream : s->st;
   s->timer.chained = false;    // the next timed pass does not start at a pass's end
+  s->stamp_chained = false;
   const int f = s->tlist_flip;
   s->tlist_flip ^= 1;
   HIPCHK(hipEventSynchronize(s->tlist_ev[f]));
@@ -606,6 +640,8 @@ int step_real_unfused(gpe_solve* s, int64_t nsteps, const double* tmid) {
         launch_kinetic_points<double>(s->psi, s->kin[0], s->kin[1], s->kin[2], s->ka, s->n, s->masked,
                                       s->kcut * s->kcut, s->st);
       HIPCHK(hipGetLastError());
+      s->timer.chained = false;
+      s->stamp_chained = false;
       CHK(pass(s, K_X_INV, false, 0, 0, 0));
       CHK(pass(s, K_Y_INV, false, 0, 0, 0));
       CHK(pass(s, K_Z_INV, false, 0, 0, 0));
@@ -903,7 +939,7 @@ int gpe_solve_destroy(gpe_solve* s) {
                   s->m_vol[0], s->m_vol[1], s->w_xy, s->w_z, s->w_vol[0], s->w_vol[1], s->d_tab[0], s->d_tab[1],
                   s->d_tab[2], s->g_tab[0], s->g_tab[1], s->g_tab[2], s->d_tlist, s->red, s->counter, s->scal,
                   s->hist, s->nkocc, s->nkcnt, s->d_out, s->d_nkocc, s->d_nkcnt, s->stage, s->desc_step,
-                  s->desc_diag};
+                  s->desc_diag, s->stamps};
   for (void* p : ptrs) dfree(p);
   for (auto& p : s->dyn) {
     dfree(p.code);
@@ -1099,6 +1135,51 @@ int gpe_step_unfused(gpe_solve* s, int64_t nsteps, const double* t_mid) {
     return fail(GPE_ERR_INVALID, "t_mid is required: a time-dependent potential is registered");
   CHK(set_device(s));
   return step_real_unfused(s, nsteps, t_mid);
+}
+
+int gpe_step_stamped(gpe_solve* s, int64_t nsteps, const double* t_mid, double* us_per_kind,
+                     int64_t* launches_per_kind) {
+  if (!s || !us_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(set_device(s));
+  int rate_khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, s->ctx->device));
+  if (rate_khz <= 0) return fail(GPE_ERR_HIP, "the device reports no wall-clock rate");
+  if (!s->stamps) {
+    CHK(dmalloc((void**)&s->stamps, kMaxStamps * sizeof(unsigned long long)));
+    CHK(refresh_desc(s));
+  }
+  launch_clear_stamps(s->stamps, kMaxStamps, s->st);
+  HIPCHK(hipGetLastError());
+  s->stamp_spans.clear();
+  s->stamp_used = 0;
+  s->stamp_chained = false;
+  s->stamping = true;
+  const int r = do_step(s, nsteps, t_mid);
+  s->stamping = false;
+  s->stamp_chained = false;
+  CHK(r);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned long long> h(s->stamp_used);
+  if (!h.empty())
+    HIPCHK(hipMemcpyAsync(h.data(), s->stamps, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int k = 0; k < kPassKinds; ++k) {
+    us_per_kind[k] = 0.0;
+    launches_per_kind[k] = 0;
+  }
+  auto latest = [&](long long first, long long count) {
+    unsigned long long t = 0;
+    for (long long i = first; i < first + count; ++i) t = std::max(t, h[i]);
+    return t;
+  };
+  for (const auto& span : s->stamp_spans) {
+    const unsigned long long begin = latest(span.begin, span.begin_count), end = latest(span.end, span.end_count);
+    if (end < begin) return fail(GPE_ERR_STATE, "a stamped launch ended before its span began");
+    us_per_kind[span.kind] += double(end - begin) * 1e3 / rate_khz;
+    launches_per_kind[span.kind] += 1;
+  }
+  return GPE_OK;
 }
 
 int gpe_step_timed(gpe_solve* s, int64_t nsteps, const double* t_mid, double* ms_per_kind,
@@ -1299,7 +1380,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 }
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
-  PassArgsRaw a{b->desc, sa, sb, flags, hist};
+  PassArgsRaw a{b->desc, sa, sb, flags, hist, 0};
   hipEvent_t stop = nullptr;
   if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &stop));
   hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, nullptr, stop);
@@ -1481,6 +1562,7 @@ int gpe_mark(gpe_solve* s, int32_t which) {
   if (!s->mark[which]) HIPCHK(hipEventCreate(&s->mark[which]));
   HIPCHK(hipEventRecord(s->mark[which], s->st));
   s->timer.chained = false;               // a pass timed next starts after the mark
+  s->stamp_chained = false;
   return GPE_OK;
 }
 

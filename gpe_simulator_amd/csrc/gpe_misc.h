@@ -38,6 +38,8 @@ void launch_finish_sums(const double* red, long long nblk, int nq, double* out, 
 void launch_finish_nk(const double* occ, const unsigned int* cnt, long long nblk, int nbins, double* out_occ,
                       long long* out_cnt, hipStream_t st);
 void launch_copy(const void* in, void* out, long long bytes, hipStream_t st);
+void launch_clear_stamps(unsigned long long* stamps, long long count, hipStream_t st);
+void launch_stamp_now(unsigned long long* slot, hipStream_t st);
 
 // Pass launchers: one per (precision, side, variant) instantiation; side 0 is
 // the generic transform (one variant).

@@ -175,6 +175,17 @@ __global__ void k_kinetic_points(cpx<R>* psi, const cpx<double>* kx, const cpx<d
   }
 }
 
+// Launch-timing slots start at 0 (raised by atomicMax).
+__global__ void k_clear_stamps(unsigned long long* stamps, long long count) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    stamps[e] = 0ULL;
+}
+
+// A reference time on the stream: the wall clock when this one-thread launch
+// runs, i.e. when the work enqueued before it has finished.
+__global__ void k_stamp_now(unsigned long long* slot) { *slot = (unsigned long long)wall_clock64(); }
+
 template <typename R>
 __global__ void k_scale(cpx<R>* psi, long long count, const double* scal) {
   const double s = scal[1];
@@ -337,6 +348,12 @@ void launch_add_f64(double* out, const double* in, long long size, hipStream_t s
 template <typename R>
 void launch_scale(void* psi, long long count, const double* scal, hipStream_t st) {
   hipLaunchKernelGGL(k_scale<R>, dim3(nblocks_stride(count, 256)), dim3(256), 0, st, (cpx<R>*)psi, count, scal);
+}
+void launch_clear_stamps(unsigned long long* stamps, long long count, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_stamps, dim3(nblocks_stride(count, 256)), dim3(256), 0, st, stamps, count);
+}
+void launch_stamp_now(unsigned long long* slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_stamp_now, dim3(1), dim3(1), 0, st, slot);
 }
 template <typename R>
 void launch_kinetic_points(void* psi, const void* kx, const void* ky, const void* kz, const double* karr, int n,

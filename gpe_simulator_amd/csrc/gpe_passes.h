@@ -52,6 +52,7 @@ enum PassFlags : int {
   F_OUT_WORK = 4,  // write to the workspace instead of psi
   F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
   F_SPECTRUM = 16, // X_DIAG: also bin |F|^2 into the shells and the view
+  F_STAMP = 32,    // record the launch's end on the device wall clock
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -75,6 +76,7 @@ struct SolveDev {
   double* red;             // per-block partial sums
   double* scal;            // [0] norm, [1] 1/sqrt(norm)
   double* hist;            // norm history per step (nullable)
+  unsigned long long* stamps;  // wall-clock slots: block ends and reference times (F_STAMP)
   double* nkocc;           // [block][nbins] |F|^2 partials
   unsigned int* nkcnt;     // [block][nbins] mode-count partials
   double dx3;              // dx*dx*dx
@@ -95,6 +97,7 @@ struct PassArgs {
   int slot_a, slot_b;
   int flags;
   int hist_index;          // step index into s->hist (Z_LAST with F_NORM)
+  int stamp_index;         // first of this launch's per-block end slots in s->stamps (F_STAMP)
 };
 
 // ------------------------------------------------------------ transforms --
@@ -361,6 +364,18 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
   constexpr int M = TF::M;
+  // Launch timing from the device's constant-rate wall clock: when a wave is
+  // done, its first lane raises its block's slot to the time (an atomic with
+  // no return, so the wave does not wait; no barrier, which would make every
+  // wave wait for its stores).  One slot per block: a single slot for the
+  // launch would serialise thousands of atomics in L2.  The host takes the
+  // latest block end as the launch's end, and a launch's span from the
+  // previous timed launch's end (or a reference time) to its own, with no
+  // timing events between launches.  The clock is read only at the end: the
+  // clock intrinsic counts as a side effect, and one before the descriptor
+  // reads would keep the compiler from reading the descriptor through the
+  // scalar unit.
+  const bool stamp = (a.flags & F_STAMP) != 0;
 
   int t, b;
   if constexpr (AXIS == 2) {
@@ -663,6 +678,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       s.nkcnt[(long long)bid * nb + q] = hcnt[q];
     }
   }
+  if (stamp && (threadIdx.x & 63) == 0)
+    atomicMax(s.stamps + a.stamp_index + blockIdx.x, (unsigned long long)wall_clock64());
 }
 
 // Sums the Z_LAST norm partials of one solve per block (blockIdx.x = member),

@@ -250,6 +250,17 @@ int gpe_mark_elapsed(gpe_solve* solve, double* ms);
  * gpe_batch_step_timed does the same for a batch (eager launches, no graph). */
 int gpe_step_timed(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* ms_per_kind,
                    int64_t* launches_per_kind);
+/* gpe_step with each pass launch timed by the device's constant-rate wall
+ * clock instead of events: every wave of a launch records its end time, the
+ * latest is the launch's end, and a launch's span runs from the previous
+ * timed launch's end (or from a reference time taken just before it, when
+ * other work such as a time-dependent table evaluation came in between) to
+ * its own end: completion to completion, with no events between the
+ * launches.  Reports summed microseconds per pass kind and launch counts
+ * like gpe_step_timed.  Launches beyond 4M block slots per call run untimed
+ * (about 1000 launches at 256^3). */
+int gpe_step_stamped(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* us_per_kind,
+                     int64_t* launches_per_kind);
 /* Device-to-device streaming copy of `bytes` (multiple of 16), repeated
  * `reps` times; returns milliseconds per copy.  Bandwidth calibration. */
 int gpe_bench_copy(gpe_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy);

@@ -334,3 +334,29 @@ def test_reference_structured_step_is_real_time_only(require_gpu):
     with pytest.raises(_lib.GpeInvalid, match="real time only"):
         solve.step_unfused(2)
     solve.close()
+
+
+def test_stamped_step_matches_the_plain_step_and_reports_every_launch(require_gpu):
+    """gpe_step_stamped advances the state exactly as gpe_step and reports
+    one positive span per pass launch (time-dependent tables break the chain
+    of spans and are not charged to any pass)."""
+    from gpe_simulator_amd.engine import midpoint_times
+    n, length, dt, steps = 64, 12.0, 1e-3, 7
+    beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)"
+    _, stamped = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
+    _, plain = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
+    psi0 = random_state(n, length, seed=3)
+    tm = midpoint_times(0, steps, dt)
+    for solve in (stamped, plain):
+        solve.upload(psi0)
+    spans = stamped.step_stamped(steps, tm)
+    plain.step(steps, tm)
+    np.testing.assert_array_equal(stamped.to_numpy(), plain.to_numpy())
+    counts = {k: v[1] for k, v in spans.items() if v[1]}
+    assert counts == {"z_first": 1, "z_mid": steps - 1, "z_last": 1, "y_fwd": steps, "y_inv": steps,
+                      "x_kin": steps}
+    for kind, (us, count) in spans.items():
+        if count:
+            assert 0 < us / count < 5e3, (kind, us, count)
+    stamped.close()
+    plain.close()
