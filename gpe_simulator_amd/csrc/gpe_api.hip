@@ -369,7 +369,7 @@ int ensure_red(gpe_solve* s, long long need) {
 
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
   int stamp_index = 0;
-  const long long nblk = s->blocks(kind, s->n);
+  const long long nblk = s->stamping && kind < kPassKinds ? s->blocks(kind, s->n) : 0;
   if (s->stamping && kind < kPassKinds && s->stamp_used + 1 + nblk <= kMaxStamps) {
     // the span begins at the previous timed launch's block ends, or at a
     // reference time when other work ran since
