@@ -122,3 +122,32 @@ def test_variant_out_of_range_is_rejected(require_gpu):
         s.set_variants([99] * len(_lib.PASS_KINDS))
     with pytest.raises(AssertionError):
         s.set_variants([0] * 3)
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-12), ("c64", 2e-5)])
+@pytest.mark.parametrize("n", [96, 256])
+def test_reference_structured_step_under_every_variant(n, precision, tol, require_gpu):
+    """The plain pass kinds of gpe_step_unfused (half-step alone, inverse FFT
+    along z, FFT and inverse FFT along x) under every compiled variant of the
+    side, including the two-round exchange, against the fused step."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    length, dt, steps = 16.0, 5e-4, 3
+    x, k = orc.get_grids(n, length)
+    psi0 = orc.synthetic_state(n, length, seed=2, beta=100.0)
+    tm = midpoint_times(0, steps, dt)
+    fused = make_solve(n, x, k, "real", dt, 100.0, static="x**2 + y**2 + z**2", dynamic=BEAM,
+                       precision=precision)
+    fused.set_variants([0] * len(_lib.PASS_KINDS))
+    fused.upload(psi0)
+    fused.step(steps, tm)
+    expected = fused.to_numpy()
+    fused.close()
+    for v in range(variant_count(n, precision)):
+        solve = make_solve(n, x, k, "real", dt, 100.0, static="x**2 + y**2 + z**2", dynamic=BEAM,
+                           precision=precision)
+        solve.set_variants([v] * len(_lib.PASS_KINDS))
+        solve.upload(psi0)
+        solve.step_unfused(steps, tm)
+        assert rel_l2(solve.to_numpy(), expected) < tol, (n, precision, v)
+        solve.close()
