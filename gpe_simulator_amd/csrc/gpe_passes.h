@@ -117,11 +117,36 @@ struct FastTF {
   // pencils are consecutive, so each wave belongs to one half.
   static constexpr int ROUNDS = RD_;
   static_assert(ROUNDS == 1 || (ROUNDS == 2 && (B / 2) % (64 / TP) == 0), "a round must hold whole waves");
-  static constexpr int LSTRIDE = TP + 1;           // padded LDS row
-  // Odd pencil stride: in the X/Y passes consecutive lanes own consecutive
-  // pencils, so an even stride would put them on the same LDS bank.
-  static constexpr int PS = M * LSTRIDE + ((M * LSTRIDE) % 2 == 0 ? 1 : 0);
-  static constexpr int LDS_ELEMS = B * PS;
+  // LDS exchange layout: a pencil's M rows of TP values, row_stride<AXIS>()
+  // apart, pencils pencil_stride<AXIS>() apart.
+  // Z passes: a pencil's TP threads are adjacent lanes, and in the transpose
+  // read lane t reads row t + TP*q, so rows are padded to TP + 1 to put
+  // those lanes on different banks.
+  // X/Y passes: adjacent lanes own adjacent pencils at the same t, and an
+  // odd pencil stride puts up to 256 bytes of such pencils on distinct
+  // banks (an even stride would put neighbours on the same bank).  When a
+  // block's pencils span those 256 bytes (B * sizeof(complex) >= 256), every
+  // group of lanes the LDS serves at once is pencils at one t, so the rows
+  // need no padding; with fewer pencils a group spans two t and the padded
+  // rows keep them apart.  Unpadded rows save a ninth to a fifth of the
+  // exchange buffer, which in the Y passes at side 192 fits one more block
+  // per CU (about 10% faster per launch).  The X passes keep the padding:
+  // the smaller buffer there raises the occupancy the compiler derives from
+  // LDS, it then holds the kinetic pass to the variant's register budget,
+  // and at side 96 (complex64, 3 waves) that pass spills and runs 30% slower.
+  static constexpr bool kPackedY = B * int(sizeof(cpx<R>)) >= 256;
+  template <int AXIS>
+  __host__ __device__ static constexpr int row_stride() {
+    return AXIS == 1 && kPackedY ? TP : TP + 1;
+  }
+  template <int AXIS>
+  __host__ __device__ static constexpr int pencil_stride() {
+    return M * row_stride<AXIS>() + ((M * row_stride<AXIS>()) % 2 == 0 ? 1 : 0);
+  }
+  template <int AXIS>
+  __host__ __device__ static constexpr int lds_elems() {
+    return B * pencil_stride<AXIS>();
+  }
   static_assert(N % TP == 0 && M % TP == 0, "N must be TP*TP*Q");
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + TP * m; }
@@ -143,8 +168,9 @@ struct FastTF {
   // RNDS exchange rounds; `group` is the round in which this thread's
   // pencil uses the LDS (0 with one round).  Every thread passes every
   // barrier; only the LDS accesses depend on the group.
-  template <int DIR, int RNDS = 1>
+  template <int DIR, int RNDS = 1, int AXIS = 2>
   __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const TwT* twl, int, int group = 0) {
+    constexpr int LSTRIDE = row_stride<AXIS>();
     dft<M, DIR, R>(v);
     static_for<1, M>([&](auto k1c) {
       constexpr int k1 = decltype(k1c)::value;
@@ -190,8 +216,14 @@ struct GenTF {
   static constexpr int THREADS = 64;
   static constexpr int MIN_WAVES = 1;
   static constexpr int ROUNDS = 1;
-  static constexpr int PS = 1024;
-  static constexpr int LDS_ELEMS = 1024;
+  template <int AXIS>
+  __host__ __device__ static constexpr int pencil_stride() {
+    return 1024;
+  }
+  template <int AXIS>
+  __host__ __device__ static constexpr int lds_elems() {
+    return 1024;
+  }
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + 64 * m; }
   __device__ static __forceinline__ bool valid(int t, int m, int n) { return t + 64 * m < n; }
@@ -199,7 +231,7 @@ struct GenTF {
   static constexpr int TWL_ELEMS = 1;
   __device__ static __forceinline__ void stage_twiddles(TwT*, const cpx<double>*) {}
 
-  template <int DIR, int RNDS = 1>
+  template <int DIR, int RNDS = 1, int AXIS = 2>
   __device__ static __forceinline__ void fft(cpx<R>* v, cpx<R>* lp, int t, const cpx<double>* __restrict__ tw,
                                              int n, int = 0) {
     __syncthreads();
@@ -359,7 +391,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // Only Z passes exchange in rounds: there a wave's lanes own whole
   // consecutive pencils; in the X/Y passes a pencil's threads span waves.
   constexpr int kRounds = AXIS == 2 ? TF::ROUNDS : 1;
-  __shared__ cpx<R> lds[TF::LDS_ELEMS / kRounds];
+  __shared__ cpx<R> lds[TF::template lds_elems<AXIS>() / kRounds];
   __shared__ typename TF::TwT twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
@@ -388,7 +420,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const long long stride = AXIS == 2 ? 1LL : (AXIS == 1 ? (long long)n : (long long)n * n);
   const int ntiles = tile_count<TF, AXIS>(n);
   const int group = b / (TF::B / kRounds);
-  cpx<R>* lp = lds + (b % (TF::B / kRounds)) * TF::PS;
+  cpx<R>* lp = lds + (b % (TF::B / kRounds)) * TF::template pencil_stride<AXIS>();
   // Block-shared tables staged next to the twiddles (the twiddle barrier
   // covers them): the FP64 x kinetic factors for X_KIN and the z potential
   // line(s) for the Z passes, so they are read from LDS where used instead of
@@ -468,11 +500,11 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     bool write = true;
 
     if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD || KIND == K_X_FWD) {
-      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_INV || KIND == K_Z_INV || KIND == K_X_INV) {
-      TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_X_KIN) {
-      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
       if (s.imag) {
         const double eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
 #pragma unroll
@@ -505,10 +537,10 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST || KIND == K_Z_HALF) {
       if constexpr (KIND == K_Z_MID || KIND == K_Z_LAST) {
-        TF::template fft<+1, kRounds>(v, lp, t, twp, n, group);
+        TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
       } else if (prescale) {
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
@@ -556,7 +588,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       } else {
         points(std::false_type{}, std::false_type{});
       }
-      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Z_DIAG) {
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {
@@ -578,7 +610,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           acc[2] += a2 * a2;
         }
       }
-      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
       // Angular momentum by Parseval along one axis:
       //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
@@ -586,7 +618,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
-      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
       const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);
       double part = 0.0;
 #pragma unroll
@@ -599,7 +631,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       acc[0] += coord * part;
       write = false;
     } else if constexpr (KIND == K_X_DIAG) {
-      TF::template fft<-1, kRounds>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
       const double ky = gload(s.karr + pj), kz = gload(s.karr + pk);
       const int fq = freq_index(pj, n), fr = freq_index(pk, n);
       const float view_scale = s.nview ? float(s.nview) / (1.7320508f * float(s.vh)) : 0.f;
