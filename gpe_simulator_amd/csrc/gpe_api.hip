@@ -387,6 +387,11 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     // the norm finisher follows an imaginary Z_LAST on the stream
     s->stamp_chained = !(kind == K_Z_LAST && (flags & F_NORM));
     flags |= F_STAMP;
+  } else if (s->stamping && kind < kPassKinds) {
+    // out of slots: this launch goes untimed, and a later (smaller) launch
+    // that still fits must not count it, so its span starts at a new
+    // reference time
+    s->stamp_chained = false;
   }
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index};
   hipEvent_t stop = nullptr;
