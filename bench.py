@@ -255,7 +255,7 @@ def main(argv=None):
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "the engine runs a solve on one GPU and has no multi-GPU path", file=sys.stderr)
         return 2
 
     from gpe_simulator_amd import _lib
