@@ -142,10 +142,16 @@ struct FastTF {
   __host__ __device__ static constexpr int row_stride() {
     return AXIS == 1 && kPackedY ? TP : TP + 1;
   }
+  // Z passes: the lanes served together hold whole pencils of TP lanes each,
+  // and their rows are spread over the banks when the pencils start an odd
+  // number of TP-element units apart: the smallest such stride that holds
+  // M padded rows, TP * (M + Q) elements, rounded up to an odd multiple of TP.
   template <int AXIS>
   __host__ __device__ static constexpr int pencil_stride() {
+    if (AXIS == 2) return TP * ((M + Q) % 2 == 0 ? M + Q + 1 : M + Q);
     return M * row_stride<AXIS>() + ((M * row_stride<AXIS>()) % 2 == 0 ? 1 : 0);
   }
+  static_assert(TP * (M + Q) >= M * (TP + 1), "a Z pencil stride must hold M padded rows");
   template <int AXIS>
   __host__ __device__ static constexpr int lds_elems() {
     return B * pencil_stride<AXIS>();
