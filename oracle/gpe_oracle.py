@@ -15,7 +15,13 @@ time reversibility, Parseval, a brute-force DFT at N <= 6, the harmonic
 ground-state energy E = mu = 3, plane-wave mu and E, the integer shell
 identity and the stage clock t = i*dt (tests/test_workflow.py:262).  The
 equal-width spectrum view (no reference counterpart) is pinned against a
-50-digit decimal evaluation of its defining formula.
+50-digit decimal evaluation of its defining formula.  The Gaussian start
+against the continuum normalisation pi^(-3/4) sigma^(-3/2) and its profile,
+beta against the workload table (beta = 200, 1000, 1100, 4000 at the listed
+scattering lengths), every expression function against Python's math, the
+box and absorber volumes against hand-evaluated points (wall maxima, face
+ramps, corner sums), and the monitor images against direct sums, Parseval
+and symmetry.
 """
 
 from __future__ import annotations
@@ -112,6 +118,62 @@ def eval_expression(source, x, t=0.0):
         return np.power(a, b)
 
     return np.broadcast_to(ev(tree), (len(x),) * 3).astype(np.float64)
+
+
+# ----------------------------------------------------- static volumes --
+def box_potential(x, box_radius, box_length, wall_thickness, trap_depth):
+    """Cylindrical box walls: depth * max(tube, sheet) with
+    tube = exp(-(sqrt(x^2+y^2) - R)^2 / (2 w^2)) and
+    sheet = exp(-(|z| - L/2)^2 / (2 w^2)).  gpe/potentials.py:3-12."""
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    rho = np.sqrt(X * X + Y * Y)
+    tube = np.exp(-(rho - box_radius) ** 2 / (2 * wall_thickness ** 2))
+    sheet = np.exp(-(np.abs(Z) - box_length / 2) ** 2 / (2 * wall_thickness ** 2))
+    return trap_depth * np.maximum(tube, sheet)
+
+
+def absorption_potential(activation_distance, amplitude, x, simulation_range):
+    """Linear ramps inside `activation_distance` of each of the six faces of
+    the box [-range/2, range/2)^3, summed and scaled by `amplitude`.
+    gpe/potentials.py:14-28."""
+    def ramps(c):
+        return (np.maximum(activation_distance - (c + simulation_range / 2), 0)
+                + np.maximum(activation_distance - (simulation_range / 2 - c), 0))
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    return amplitude * ((ramps(X) + ramps(Y)) + ramps(Z))
+
+
+def total_static_potential(real_grid, imag_grid):
+    """Complex static volume real - i*imag (the absorber enters with a
+    negative imaginary part).  gpe/potentials.py:31-37."""
+    return np.asarray(real_grid, dtype=np.float64) - 1j * np.asarray(imag_grid, dtype=np.float64)
+
+
+# --------------------------------------------------------- monitor images --
+def monitor_images(psi, x, k):
+    """The monitor arrays of gpe/save.py:59-90 from the completed psi:
+    TOF projections sum_abs2(fftn(psi)) over z and over x times dx^3 dk /
+    (size dk^3), the kz = 0 and kx = 0 momentum slices times dx^3 / (size
+    dk^3), all fftshifted; the real-space column density sum_z |psi|^2 dx;
+    the mid-z plane's density and phase (NaN where the density is below 1e-12
+    of its maximum).  sum_abs2 is gpe/helper.py:3-12."""
+    n = psi.shape[0]
+    dx = float(x[1] - x[0])
+    dk = abs(float(k[1] - k[0]))
+    psi_k = _fftn(psi)
+    power = psi_k.real ** 2 + psi_k.imag ** 2
+    scale = dx ** 3 / (psi.size * dk ** 3)
+    shift = np.fft.fftshift
+    plane = psi[:, :, n // 2]
+    density = plane.real ** 2 + plane.imag ** 2
+    return dict(
+        TOF_xy=shift(power.sum(axis=2)) * (scale * dk),
+        TOF_yz=shift(power.sum(axis=0)) * (scale * dk),
+        TOF_slice_xy=shift(power[:, :, 0]) * scale,
+        TOF_slice_yz=shift(power[0, :, :]) * scale,
+        density_xy=(psi.real ** 2 + psi.imag ** 2).sum(axis=2) * dx,
+        density_slice_xy=density,
+        phase_xy=np.where(density > density.max() * 1e-12, np.angle(plane), np.nan))
 
 
 # ------------------------------------------------------------ local steps --

@@ -317,3 +317,160 @@ def test_spectrum_view_totals_and_limits(n):
     expected = orc.spectrum_view_bin(*mode, n)
     assert view["occupation"][expected] == pytest.approx(np.sum(np.abs(np.fft.fftn(wave)) ** 2), rel=1e-12)
     assert view["occupation"].sum() - view["occupation"][expected] < 1e-12 * view["occupation"][expected]
+
+
+# ----------------------------------------------- initial state, beta, VM --
+def test_gaussian_unit_width_is_the_oscillator_ground_state():
+    """sigma = 1 on [-10, 10)^3, 64^3: the normalised discrete Gaussian equals
+    the continuum ground state pi^(-3/4) exp(-r^2/2) (the grid sum of a
+    Gaussian this deep inside the box is its integral to rounding)."""
+    x, _ = orc.get_grids(64, 20.0)
+    psi = orc.gaussian(x, 1.0)
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    exact = np.pi ** -0.75 * np.exp(-(X * X + Y * Y + Z * Z) / 2)
+    assert psi.dtype == np.complex128
+    assert np.max(np.abs(psi - exact)) < 1e-13 * np.max(exact)
+    assert orc.get_N(psi, x) == pytest.approx(1.0, abs=1e-13)
+
+
+@pytest.mark.parametrize("sigma", [0.8, 1.3])
+def test_gaussian_peak_and_profile(sigma):
+    """Peak (pi sigma^2)^(-3/4) at the origin (a grid point) and the axis
+    profile exp(-x^2 / (2 sigma^2)): a sigma^2 in place of 2 sigma^2, or a
+    missing normalisation, fails one of them."""
+    x, _ = orc.get_grids(64, 20.0)
+    psi = orc.gaussian(x, sigma)
+    c = 32
+    assert x[c] == 0.0
+    assert psi[c, c, c].real == pytest.approx((np.pi * sigma ** 2) ** -0.75, rel=1e-12)
+    for i in (c + 3, c - 5, c + 8):
+        ratio = (psi[i, c, c] / psi[c, c, c]).real
+        assert ratio == pytest.approx(math.exp(-x[i] ** 2 / (2 * sigma ** 2)), rel=1e-12)
+        assert psi[c, i, c] == psi[i, c, c] == psi[c, c, i]
+    assert np.all(psi.imag == 0)
+
+
+@pytest.mark.parametrize("beta, a", [(200.0, 1.0025308467604033), (1000.0, 5.012654233802016),
+                                     (1100.0, 5.513919657182218), (4000.0, 20.050616935208065)])
+def test_beta_of_the_workload_table(beta, a):
+    """beta = 8 pi N_atom a a0 / 1 um at N_atom = 150000 reproduces the
+    table of scattering lengths the workloads are set with (beta = 2g for
+    g = 100, 500, 550, 2000), and the constant 1.3299673896737356e-3 per
+    atom and unit of a."""
+    assert orc.get_beta(150000, a) == pytest.approx(beta, rel=1e-13)
+    assert orc.get_beta(1.0, 1.0) == pytest.approx(1.3299673896737356e-3, rel=1e-15)
+    assert orc.get_beta(300000, a) == pytest.approx(2 * beta, rel=1e-13)
+
+
+@pytest.mark.parametrize("func", ["sin", "cos", "tan", "exp", "sqrt", "tanh", "abs", "log"])
+def test_expression_functions_match_math(func):
+    """Every function of the grammar (config_editor/expressions.py:8) maps to
+    the C function of the same value (abs -> fabs): checked against Python's
+    math at points of the axis, on an argument that keeps sqrt and log real."""
+    x, _ = orc.get_grids(8, 4.0)
+    arg = "x + 3" if func in ("sqrt", "log") else "x"
+    got = orc.eval_expression(f"{func}({arg})", x)
+    ref = {"abs": abs}.get(func) or getattr(math, func)
+    for i in range(8):
+        val = x[i] + 3 if func in ("sqrt", "log") else x[i]
+        assert got[i, 0, 0] == pytest.approx(ref(val), rel=1e-15, abs=1e-300)
+        assert got[i, 5, 2] == got[i, 0, 0]          # depends on x only
+
+
+def test_expression_operators_constants_and_axes():
+    """Binary operators, ** as pow, unary minus, pi, t and the axis roles
+    (x first index, z last) against direct evaluation."""
+    x, _ = orc.get_grids(6, 3.0)
+    t = 0.37
+    got = orc.eval_expression("-(x**3) + 2**y / (z - 7) * pi - t*cos(pi*t)", x, t=t)
+    for i, j, k in [(0, 0, 0), (1, 4, 2), (5, 2, 3)]:
+        ref = -(x[i] ** 3) + 2 ** x[j] / (x[k] - 7) * math.pi - t * math.cos(math.pi * t)
+        assert got[i, j, k] == pytest.approx(ref, rel=1e-14)
+    assert orc.eval_expression("3", x).shape == (6, 6, 6)
+    assert np.all(orc.eval_expression("-2.5", x) == -2.5)
+
+
+# ----------------------------------------------------------- static volumes --
+def test_box_potential_at_hand_evaluated_points():
+    """Walls of a cylinder of radius 5 and length 6 (thickness 1, depth 40) on
+    a 32^3 grid of spacing 0.5: depth on the tube (rho = 5 at (5,0,z) and
+    (3,4,z)) and on the end sheets (|z| = 3), max(tube, sheet) elsewhere."""
+    x, _ = orc.get_grids(32, 16.0)
+    v = orc.box_potential(x, 5.0, 6.0, 1.0, 40.0)
+    idx = {float(val): i for i, val in enumerate(x)}
+    at = lambda px, py, pz: v[idx[px], idx[py], idx[pz]]      # noqa: E731
+    assert at(5.0, 0.0, 0.0) == pytest.approx(40.0, rel=1e-15)
+    assert at(3.0, 4.0, 0.5) == pytest.approx(40.0, rel=1e-15)
+    assert at(0.0, 0.0, 3.0) == pytest.approx(40.0, rel=1e-15)
+    assert at(0.0, 0.0, -3.0) == pytest.approx(40.0, rel=1e-15)
+    # centre: tube exp(-25/2) < sheet exp(-9/2)
+    assert at(0.0, 0.0, 0.0) == pytest.approx(40.0 * math.exp(-4.5), rel=1e-14)
+    # rho = 1.5, z = 0: tube exp(-3.5^2/2) vs sheet exp(-9/2)
+    assert at(1.5, 0.0, 0.0) == pytest.approx(40.0 * max(math.exp(-3.5 ** 2 / 2), math.exp(-4.5)), rel=1e-14)
+    # outside the tube and past the sheet: both walls decay
+    r = math.hypot(7.0, 6.0)
+    assert at(7.0, 6.0, 7.0) == pytest.approx(
+        40.0 * max(math.exp(-(r - 5) ** 2 / 2), math.exp(-(7 - 3) ** 2 / 2)), rel=1e-14)
+    assert np.all(v <= 40.0) and np.all(v > 0)
+
+
+def test_absorber_ramps_at_faces_edges_and_corners():
+    """Ramps of width 2 and amplitude 3 on [-8, 8)^3 (spacing 0.5): zero
+    inside, 3*2 on the first plane of each axis, 3*1.5 on the last (x = 7.5
+    lies 0.5 inside the face), and the sum over the faces a point is near."""
+    x, _ = orc.get_grids(32, 16.0)
+    a = orc.absorption_potential(2.0, 3.0, x, 16.0)
+    c = 16                                   # x = 0
+    assert a[c, c, c] == 0.0
+    assert a[0, c, c] == a[c, 0, c] == a[c, c, 0] == pytest.approx(6.0)
+    assert a[31, c, c] == a[c, 31, c] == a[c, c, 31] == pytest.approx(4.5)
+    assert a[1, c, c] == pytest.approx(3.0 * 1.5)          # x = -7.5
+    assert a[4, c, c] == 0.0                                # x = -6: the ramp ends
+    assert a[0, 0, c] == pytest.approx(12.0)               # edge: two faces
+    assert a[0, 0, 0] == pytest.approx(18.0)               # corner: three faces
+    assert a[0, 31, 0] == pytest.approx(6.0 + 4.5 + 6.0)
+    assert np.all(a >= 0)
+
+
+def test_total_static_potential_has_a_negative_imaginary_absorber():
+    re = np.full((2, 2, 2), 1.5)
+    im = np.full((2, 2, 2), 0.25)
+    v = orc.total_static_potential(re, im)
+    assert v.dtype == np.complex128
+    assert np.all(v.real == 1.5) and np.all(v.imag == -0.25)
+
+
+# ------------------------------------------------------------ monitor images --
+def test_monitor_images_sums_and_modes():
+    """TOF projections and the column density each integrate to the norm
+    (Parseval); a single plane-wave mode lands at its fftshifted indices; the
+    kz = 0 slice of a mode with kz != 0 is empty; the mid plane holds the
+    field's own density and phase."""
+    n, length = 16, 2 * np.pi
+    x, k = orc.get_grids(n, length)
+    dx, dk = x[1] - x[0], abs(k[1] - k[0])
+    X, Y, Z = x[:, None, None], x[None, :, None], x[None, None, :]
+    psi = np.exp(1j * (2 * X - 3 * Y + 1 * Z)) / length ** 1.5
+    img = orc.monitor_images(psi, x, k)
+    norm = orc.get_N(psi, x)
+    assert np.sum(img["TOF_xy"]) * dk * dk == pytest.approx(norm, rel=1e-12)
+    assert np.sum(img["TOF_yz"]) * dk * dk == pytest.approx(norm, rel=1e-12)
+    assert np.sum(img["density_xy"]) * dx * dx == pytest.approx(norm, rel=1e-12)
+    ks = np.fft.fftshift(k)
+    i, j = int(np.argmin(np.abs(ks - 2))), int(np.argmin(np.abs(ks + 3)))
+    peak = np.unravel_index(np.argmax(img["TOF_xy"]), img["TOF_xy"].shape)
+    assert peak == (i, j)
+    assert img["TOF_xy"][i, j] == pytest.approx(norm / dk ** 2, rel=1e-12)
+    assert np.max(img["TOF_slice_xy"]) < 1e-25
+    assert np.max(img["TOF_slice_yz"]) < 1e-25       # kx = 2 != 0
+    np.testing.assert_allclose(img["density_slice_xy"], np.abs(psi[:, :, n // 2]) ** 2, rtol=1e-14)
+    np.testing.assert_allclose(img["phase_xy"], np.angle(psi[:, :, n // 2]), rtol=1e-14, atol=1e-15)
+    # a centred Gaussian: the projections are symmetric under k -> -k
+    g = orc.gaussian(x, 0.7)
+    gi = orc.monitor_images(g, x, k)
+    tof = gi["TOF_xy"][1:, 1:]                        # drop the unpaired Nyquist row and column
+    np.testing.assert_allclose(tof, tof[::-1, ::-1], rtol=1e-10, atol=1e-16)
+    plane = np.abs(g[:, :, n // 2]) ** 2
+    dark = plane <= plane.max() * 1e-12
+    assert dark.any() and not dark.all()
+    assert np.array_equal(np.isnan(gi["phase_xy"]), dark)
