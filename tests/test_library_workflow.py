@@ -115,6 +115,35 @@ def test_duplicate_and_escaping_output_folders(tmp_path):
         parse_yaml(path)
 
 
+def test_stage_folders_names_and_load_paths(tmp_path):
+    """Default folders are stage_NN plus the name reduced to [A-Za-z0-9_-]
+    runs, under one run stamp in both roots; an absolute save.directory holds
+    the data itself while the monitor files keep the default folder; a
+    relative load path is taken from the YAML file's folder; noise and
+    switch groups arrive with the stage's keyword names."""
+    from gpe_simulator_amd.configuration import dump_yaml
+    from gpe_simulator_amd.yaml_parser import parse_yaml
+    path, config = write_config(tmp_path, stages=3)
+    config["stages"][0]["name"] = "  Ground state: g=100!  "
+    config["stages"][1]["name"] = "***"
+    config["stages"][2]["save"]["directory"] = str(tmp_path / "elsewhere")
+    config["stages"][0]["initial_state"].update(type="load", path="states/psi.npy")
+    path.write_text(dump_yaml(config))
+    first, second, third = parse_yaml(path)
+    stamp = first.data_dir.parent.name
+    assert first.name == "Ground state: g=100!" and second.name == "***"
+    assert first.data_dir == tmp_path / "data" / stamp / "stage_01_Ground_state_g_100"
+    assert first.monitor_dir == tmp_path / "monitor" / stamp / "stage_01_Ground_state_g_100"
+    assert second.data_dir.name == "stage_02" and second.data_dir.parent.name == stamp
+    assert third.data_dir == tmp_path / "elsewhere" / stamp
+    assert third.monitor_dir == tmp_path / "monitor" / stamp / "stage_03_Preparation"   # the default name
+    assert first.initial_state["path"] == str(tmp_path / "states" / "psi.npy")
+    assert "fraction" in first.noise_params and "fraction_of_trap_depth" not in first.noise_params
+    assert first.box_params["activate"] is False and "enabled" not in first.box_params
+    assert first.k_cut_params == {"activate": config["stages"][0]["k_cut"]["enabled"],
+                                  "k_cut": config["stages"][0]["k_cut"]["value"]}
+
+
 def test_stage_clock_and_per_step_propagator(tmp_path):
     """A propagator without `advance` is called once per step with t = i*dt,
     and the clock restarts in each stage (tests/test_workflow.py:235-272)."""
