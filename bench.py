@@ -12,8 +12,10 @@ HBM before timing starts.
 Beside it, the same line reports: the complex128 run of the same workload,
 the same complex128 run with the reference's launch structure (one launch per
 reference kernel and transform axis, 9 sweeps of psi per step instead of 4),
-per-kernel launch timings (device wall clock; HIP events beside them), the
-oracle on the host as the CPU baseline, a
+per-kernel launch durations (each dispatch's own start and end timestamps),
+the oracle on the host as the CPU baseline (one thread, and a threaded run
+whose final state is the FP64 reference the complex64 and complex128 engines
+are held to after the same steps), a
 short seeded serving stream (p50/p99 time-to-result), the 64-solve 96^3
 batched ground-state sweep against sequential complex128 solves, and a
 16-speed stirring sweep at 128^3 batched against the same runs in turn.
@@ -92,20 +94,22 @@ def run_engine(precision, psi0, x, k, steps, warmup, unfused=False):
 
 
 def kernel_timing(solve, steps, repeats=3):
-    """Per-pass launch durations over `steps` steps, two ways:
-    {kind: us per launch} from the device wall clock at each launch's end
-    (median of `repeats` runs), and {kind: us per launch} from HIP events
-    around every launch (one run; the events lengthen the launches they
-    bracket)."""
+    """Per-pass launch durations over `steps` steps of the workload, two ways:
+    {kind: us per launch} from each launch's own dispatch start and end
+    timestamps (a HIP event pair handed to the dispatch, nothing recorded
+    between launches: the duration a kernel trace reports; median of
+    `repeats` runs), and {kind: us per launch} as spans from one launch's end
+    to the next one's (chained stop events; includes the gap between
+    launches)."""
     import statistics
     from gpe_simulator_amd.engine import midpoint_times
     tm = midpoint_times(0, steps, DT)
-    runs = [solve.step_stamped(steps, tm) for _ in range(repeats)]
-    stamped = {kk: statistics.median(r[kk][0] / r[kk][1] for r in runs) for kk in runs[0] if runs[0][kk][1]}
-    launches = {kk: runs[0][kk][1] for kk in stamped}
+    runs = [solve.step_kernel_times(steps, tm) for _ in range(repeats)]
+    own = {kk: statistics.median(r[kk][0] / r[kk][1] for r in runs) for kk in runs[0] if runs[0][kk][1]}
+    launches = {kk: runs[0][kk][1] for kk in own}
     events = solve.step_timed(steps, tm)
-    evented = {kk: v[0] / v[1] * 1e3 for kk, v in events.items() if v[1]}
-    return stamped, launches, evented
+    spans = {kk: v[0] / v[1] * 1e3 for kk, v in events.items() if v[1]}
+    return own, launches, spans
 
 
 def measured_traffic(kernel):
@@ -205,8 +209,10 @@ def stirring(count, n=128, steps=500):
 
 
 def cpu_baseline(psi0, samples, workers=None):
-    """The FP64 oracle on a bounded sample: numpy.fft on one thread, or
-    scipy.fft with `workers` threads."""
+    """The FP64 oracle on a bounded sample of the workload: numpy.fft on one
+    thread, or scipy.fft with `workers` threads.  Steps 0..samples from psi0
+    (t_i = i*dt); step 0 is the untimed warm-up.  Returns (rate, elapsed,
+    final psi, the oracle's FP64 diagnostics of it)."""
     from oracle import gpe_oracle as orc
     orc.use_threads(workers)
     try:
@@ -217,14 +223,43 @@ def cpu_baseline(psi0, samples, workers=None):
         for i in range(1, samples + 1):
             psi = problem.step(psi, i * DT)
         elapsed = time.perf_counter() - t0
+        energies = problem.energies(psi, (samples + 1) * DT)
     finally:
         orc.use_threads(None)
-    return N ** 3 * samples / elapsed, elapsed
+    return N ** 3 * samples / elapsed, elapsed, psi, energies
+
+
+def accuracy_vs_oracle(psi0, x, k, steps, reference, energies):
+    """The complex64 and complex128 engines after `steps` steps of the
+    workload from psi0, against the FP64 oracle's state and diagnostics after
+    the same steps (computed in the CPU-baseline leg)."""
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    dx3 = float(x[1] - x[0]) ** 3
+    n0 = float(np.sum(np.abs(psi0) ** 2) * dx3)
+    rho_ref = np.abs(reference) ** 2
+    out = {"against": f"FP64 oracle after {steps} steps of the same workload from the same state",
+           "steps": steps, "mu_oracle": energies["mu"], "E_oracle": energies["E"],
+           "norm_drift_oracle": abs(energies["norm"] - n0) / n0}
+    for precision in ("c64", "c128"):
+        solve = make_solve(N, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=precision)
+        solve.upload(psi0)
+        solve.step(steps, midpoint_times(0, steps, DT))
+        d = solve.diagnostics(steps * DT, spectrum=False)
+        psi = solve.to_numpy()
+        solve.close()
+        out[precision] = {
+            "mu_rel": abs(d["mu"] - energies["mu"]) / abs(energies["mu"]),
+            "E_rel": abs(d["E"] - energies["E"]) / abs(energies["E"]),
+            "norm_drift": abs(d["norm"] - n0) / n0,
+            "psi_rel_l2": float(np.linalg.norm(psi - reference) / np.linalg.norm(reference)),
+            "density_rel_l2": float(np.linalg.norm(np.abs(psi) ** 2 - rho_ref) / np.linalg.norm(rho_ref))}
+    return out
 
 
 def accuracy(psi0, x, run64, run128):
-    """complex64 against the complex128 engine after the same steps (the
-    complex128 path is held to the FP64 oracle by the parity tests)."""
+    """complex64 against the complex128 engine after the whole timed run (the
+    complex128 path is held to the FP64 oracle by the parity tests and by
+    accuracy_vs_oracle over the first steps)."""
     dx3 = float(x[1] - x[0]) ** 3
     n0 = float(np.sum(np.abs(psi0) ** 2) * dx3)
     (psi64, d64), (psi128, d128) = run64, run128
@@ -245,6 +280,9 @@ def main(argv=None):
     parser.add_argument("--c128-steps", type=int, default=None, help="default: --steps")
     parser.add_argument("--timed-steps", type=int, default=50)
     parser.add_argument("--cpu-samples", type=int, default=3)
+    parser.add_argument("--oracle-steps", type=int, default=25,
+                        help="steps of the threaded oracle run that times the multi-core CPU baseline and "
+                             "is the FP64 reference of accuracy_c64")
     parser.add_argument("--skip-cpu", action="store_true")
     parser.add_argument("--skip-c128", action="store_true")
     parser.add_argument("--unfused-steps", type=int, default=500,
@@ -270,16 +308,15 @@ def main(argv=None):
     value = N ** 3 * args.steps / el64
     log(f"c64: {el64 * 1e3 / args.steps:.4f} ms/step, {value:.4e} grid-pt-steps/s, mu={diag64['mu']:.9g}")
 
-    log(f"per-kernel timing over {args.timed_steps} steps (device wall clock, then HIP events)")
-    stamped, launches, evented = kernel_timing(solve64, args.timed_steps)
-    dominant = max(stamped, key=lambda kk: stamped[kk] * launches[kk])
-    dom_ms = stamped[dominant] * 1e-3
-    dom_event_ms = evented[dominant] * 1e-3
+    log(f"per-kernel timing over {args.timed_steps} steps (dispatch timestamps, then chained spans)")
+    own, launches, spans = kernel_timing(solve64, args.timed_steps)
+    dominant = max(own, key=lambda kk: own[kk] * launches[kk])
+    dom_ms = own[dominant] * 1e-3
     bytes_per_launch = N ** 3 * 2 * 8                 # one read + one write sweep of complex64 psi
     achieved = bytes_per_launch / (dom_ms * 1e-3) / 1e9
     per_kernel = {kk: {"avg_us": round(us, 3), "launches": launches[kk],
-                       "event_avg_us": round(evented[kk], 3) if kk in evented else None}
-                  for kk, us in stamped.items()}
+                       "span_avg_us": round(spans[kk], 3) if kk in spans else None}
+                  for kk, us in own.items()}
     dom_variant = _lib.pass_variants(N, _lib.C64)[solve64.variants()[_lib.PASS_KINDS.index(dominant)]]
     dom_traced_us = traced_avg_us(dominant, *dom_variant)
     solve64.close()
@@ -311,17 +348,23 @@ def main(argv=None):
             "speedup_fused_c128": v128 / vu, "speedup_c64": value / vu}
     del final64
 
-    cpu, cpu_threads = None, None
+    cpu, cpu_threads, acc_oracle = None, None, None
     if not args.skip_cpu:
         log(f"CPU oracle baseline: {args.cpu_samples} steps at {N}^3")
-        rate, el = cpu_baseline(psi0, args.cpu_samples)
+        rate, el, _, _ = cpu_baseline(psi0, args.cpu_samples)
         cpu = {"value": rate, "unit": "grid-pt-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_samples} Strang steps of the same 256^3 stirring workload "
                          f"(complex128 numpy.fft, 1 thread) after 1 warm-up step, {el:.1f} s"}
         workers = min(16, os.cpu_count() or 1)
-        rate, el = cpu_baseline(psi0, args.cpu_samples, workers=workers)
+        samples = max(1, args.oracle_steps - 1)
+        log(f"threaded CPU oracle: {samples + 1} steps at {N}^3 on {workers} threads")
+        rate, el, ref_psi, ref_energies = cpu_baseline(psi0, samples, workers=workers)
         cpu_threads = {"value": rate, "unit": "grid-pt-steps/s", "cores": workers, "kind": "port",
-                       "sample": f"the same {args.cpu_samples} steps with scipy.fft on {workers} threads, {el:.1f} s"}
+                       "sample": f"{samples} Strang steps of the same workload after 1 warm-up step, scipy.fft "
+                                 f"on {workers} threads, {el:.1f} s"}
+        log(f"engine accuracy against the oracle after {samples + 1} steps")
+        acc_oracle = accuracy_vs_oracle(psi0, x, k, samples + 1, ref_psi, ref_energies)
+        del ref_psi
 
     served = None
     if args.serve_requests > 0:
@@ -358,10 +401,10 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "algorithmic_bytes": bytes_per_launch, "avg_us": dom_ms * 1e3,
-                     "timing": "device wall clock at each launch's end, completion to completion, "
-                               "median of 3 runs of the timed steps",
-                     # the same launches timed with HIP events, which add their own cost
-                     "event_avg_us": dom_event_ms * 1e3,
+                     "timing": f"each launch's own dispatch start/end timestamps (HIP event pair on the "
+                               f"solve's stream), median over 3 runs of {args.timed_steps} steps",
+                     # launch end to launch end, gaps between launches included
+                     "span_avg_us": spans[dominant],
                      # the same variant's average in the committed kernel trace of this workload
                      "traced_avg_us": dom_traced_us,
                      "traced_frac": (bytes_per_launch / (dom_traced_us * 1e-6) / 1e9 / HBM_PEAK_GBS
@@ -373,7 +416,8 @@ def main(argv=None):
         "c128": result_c128,
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_threads,
-        "accuracy_c64": acc,
+        "accuracy_c64": acc_oracle,
+        "accuracy_c64_vs_c128": acc,
         "p50_solve_ms": served["p50_ms"] if served else None,
         "p99_solve_ms": served["p99_ms"] if served else None,
         "serving": served,

@@ -35,6 +35,7 @@ EXPORTS = (
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
+    "gpe_step_kernel_times",
 )
 
 
@@ -94,6 +95,7 @@ def _declare(lib):
         "gpe_mark_elapsed": (c_int, [vp, dp]),
         "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_step_stamped": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
+        "gpe_step_kernel_times": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
         "gpe_angular_momentum": (c_int, [vp, dp]),
         "gpe_batch_create": (c_int, [vp, POINTER(vp), i32, POINTER(vp)]),
@@ -348,6 +350,16 @@ class Solve:
         cnt = np.zeros(len(PASS_KINDS), dtype=np.int64)
         check(lib().gpe_step_stamped(self.handle, int(nsteps), dptr(tm) if tm is not None else None, dptr(us),
                                      i64ptr(cnt)))
+        return {k: (float(us[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def step_kernel_times(self, nsteps, t_mid=None):
+        """gpe_step with each launch's own dispatch start and end timestamps
+        (the durations a kernel trace reports): {kind: (summed us, launches)}."""
+        tm = None if t_mid is None else np.ascontiguousarray(t_mid, dtype=np.float64)
+        us = np.zeros(len(PASS_KINDS))
+        cnt = np.zeros(len(PASS_KINDS), dtype=np.int64)
+        check(lib().gpe_step_kernel_times(self.handle, int(nsteps), dptr(tm) if tm is not None else None,
+                                          dptr(us), i64ptr(cnt)))
         return {k: (float(us[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
 
     def set_variants(self, variants):

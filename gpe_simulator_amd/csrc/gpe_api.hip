@@ -40,6 +40,7 @@ int fail(int code, const std::string& msg) {
 constexpr int kPassKinds = 11;
 constexpr long long kDynBudget = 64LL << 20;   // bytes of time-dependent slot tables
 constexpr long long kStageElems = 1LL << 20;   // complex128 elements per staging copy
+const double kUnitScale = 1.0;                  // host source of unit pending scales
 
 struct DevProgram {
   int* code = nullptr;
@@ -114,6 +115,11 @@ struct PassTimer {
   std::vector<Span> spans;
   size_t used = 0;
   bool chained = false;        // the stream's last command was a timed pass
+  // own = true: each launch carries its own start and stop event, both
+  // timestamped by the dispatch itself, so a span is the kernel's execution
+  // alone (the figure a kernel trace reports), with nothing recorded between
+  // launches
+  bool own = false;
   double ms[kPassKinds] = {0};
   long long cnt[kPassKinds] = {0};
 
@@ -121,6 +127,7 @@ struct PassTimer {
     spans.clear();
     used = 0;
     chained = false;
+    own = false;
     for (int k = 0; k < kPassKinds; ++k) {
       ms[k] = 0;
       cnt[k] = 0;
@@ -136,15 +143,18 @@ struct PassTimer {
     *idx = (int)used++;
     return hipSuccess;
   }
-  // Before a timed launch of `kind`: the stop event to hand to the launch.
-  hipError_t begin(int kind, hipStream_t st, hipEvent_t* stop) {
-    int a = chained ? spans.back().stop : -1, b = -1;
+  // Before a timed launch of `kind`: the events to hand to the launch (the
+  // start event only in `own` mode; otherwise the span starts at the previous
+  // timed launch's stop, or at a marker recorded here).
+  hipError_t begin(int kind, hipStream_t st, hipEvent_t* start, hipEvent_t* stop) {
+    int a = !own && chained ? spans.back().stop : -1, b = -1;
     hipError_t r = hipSuccess;
-    if (a < 0 && (r = take(&a)) == hipSuccess) r = hipEventRecord(pool[a], st);
+    if (a < 0 && (r = take(&a)) == hipSuccess && !own) r = hipEventRecord(pool[a], st);
     if (r == hipSuccess) r = take(&b);
     if (r != hipSuccess) return r;
     spans.push_back({kind, a, b});
     chained = true;
+    *start = own ? pool[a] : nullptr;
     *stop = pool[b];
     return hipSuccess;
   }
@@ -233,6 +243,7 @@ struct gpe_solve {
   unsigned int* nkcnt = nullptr;
   long long nk_cap = 0;                   // blocks the nk partial buffers hold
   double* d_out = nullptr;
+  double* mon = nullptr;                  // monitor images: 5 real planes + 1 complex128 plane (7 n^2 doubles)
   double* d_nkocc = nullptr;
   long long* d_nkcnt = nullptr;
   void* stage = nullptr;
@@ -242,6 +253,7 @@ struct gpe_solve {
   std::vector<char> h_desc_step, h_desc_diag;
 
   bool pending = false;                   // psi_true = psi * scal[1]
+  bool work_is_spectrum = false;          // the work field holds fftn(psi) of the current psi
   bool timing = false;
   PassTimer timer;
   hipEvent_t mark[2] = {nullptr, nullptr};  // gpe_mark start / stop
@@ -334,6 +346,7 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
   d.kcut2 = s->kcut * s->kcut;
   d.beta = R(s->beta);
   d.hdt = R(s->dt / 2);
+  d.bhdt = R(s->beta * (s->dt / 2));
   d.n = s->n;
   d.nbins = s->nbins;
   d.nview = kViewBins;
@@ -368,6 +381,7 @@ int ensure_red(gpe_solve* s, long long need) {
 }
 
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
+  s->work_is_spectrum = false;            // every pass changes psi or the work field
   int stamp_index = 0;
   const long long nblk = s->stamping && kind < kPassKinds ? s->blocks(kind, s->n) : 0;
   if (s->stamping && kind < kPassKinds && s->stamp_used + 1 + nblk <= kMaxStamps) {
@@ -394,9 +408,9 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     s->stamp_chained = false;
   }
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index};
-  hipEvent_t stop = nullptr;
-  if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &stop));
-  hipError_t e = s->launch(kind, &a, s->n, 1, s->st, nullptr, stop);
+  hipEvent_t start = nullptr, stop = nullptr;
+  if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
+  hipError_t e = s->launch(kind, &a, s->n, 1, s->st, start, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
 }
@@ -742,6 +756,38 @@ int check_range(gpe_solve* s, int64_t offset, int64_t count) {
   return GPE_OK;
 }
 
+// Copies elements [offset, offset + count) of a field of the solve (psi or
+// the work field) to host memory as complex128, through the staging buffer
+// when the field is complex64.
+int download_field(gpe_solve* s, const void* field, double* host, int64_t offset, int64_t count) {
+  if (s->prec == GPE_C128) {
+    HIPCHK(hipMemcpyAsync(host, (const char*)field + offset * 16, count * 16, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return GPE_OK;
+  }
+  CHK(ensure_stage(s));
+  for (int64_t done = 0; done < count; done += kStageElems) {
+    const int64_t c = std::min<int64_t>(kStageElems, count - done);
+    launch_to_c128<float>((const char*)field + (offset + done) * 8, s->stage, c, s->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host + 2 * done, s->stage, c * 16, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  return GPE_OK;
+}
+
+// F = fftn(psi), raw and unnormalised like numpy's, into the work field; psi
+// keeps its values (a pending renormalisation is applied to it first).
+int transform_into_work(gpe_solve* s) {
+  CHK(ensure_diag(s));
+  CHK(materialize(s));
+  CHK(pass(s, K_Z_FWD, true, 0, 0, F_OUT_WORK));
+  CHK(pass(s, K_Y_FWD, true, 0, 0, F_IN_WORK | F_OUT_WORK));
+  CHK(pass(s, K_X_FWD, true, 0, 0, F_IN_WORK | F_OUT_WORK));
+  s->work_is_spectrum = true;
+  return GPE_OK;
+}
+
 }  // namespace
 
 // ===================================================================== ABI ==
@@ -944,7 +990,7 @@ int gpe_solve_destroy(gpe_solve* s) {
                   s->m_vol[0], s->m_vol[1], s->w_xy, s->w_z, s->w_vol[0], s->w_vol[1], s->d_tab[0], s->d_tab[1],
                   s->d_tab[2], s->g_tab[0], s->g_tab[1], s->g_tab[2], s->d_tlist, s->red, s->counter, s->scal,
                   s->hist, s->nkocc, s->nkcnt, s->d_out, s->d_nkocc, s->d_nkcnt, s->stage, s->desc_step,
-                  s->desc_diag, s->stamps};
+                  s->desc_diag, s->stamps, s->mon};
   for (void* p : ptrs) dfree(p);
   for (auto& p : s->dyn) {
     dfree(p.code);
@@ -964,6 +1010,7 @@ int gpe_solve_destroy(gpe_solve* s) {
 
 int gpe_psi_upload(gpe_solve* s, const double* host, int64_t offset, int64_t count) {
   if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  s->work_is_spectrum = false;
   CHK(check_range(s, offset, count));
   CHK(set_device(s));
   CHK(materialize(s));
@@ -1012,6 +1059,7 @@ int gpe_psi_copy(gpe_solve* dst, gpe_solve* src) {
   HIPCHK(hipStreamSynchronize(src->st));
   CHK(set_device(dst));
   dst->pending = false;
+  dst->work_is_spectrum = false;
   if (dst->prec == src->prec) {
     HIPCHK(hipMemcpyAsync(dst->psi, src->psi, src->n3 * src->esz, hipMemcpyDeviceToDevice, dst->st));
   } else if (src->prec == GPE_C64) {
@@ -1026,6 +1074,7 @@ int gpe_psi_copy(gpe_solve* dst, gpe_solve* src) {
 
 int gpe_psi_gaussian(gpe_solve* s, double sigma) {
   if (!s || !(sigma > 0)) return fail(GPE_ERR_INVALID, "sigma must be positive");
+  s->work_is_spectrum = false;
   CHK(set_device(s));
   s->pending = false;
   if (s->prec == GPE_C64)
@@ -1205,6 +1254,26 @@ int gpe_step_timed(gpe_solve* s, int64_t nsteps, const double* t_mid, double* ms
   return GPE_OK;
 }
 
+int gpe_step_kernel_times(gpe_solve* s, int64_t nsteps, const double* t_mid, double* us_per_kind,
+                          int64_t* launches_per_kind) {
+  if (!s || !us_per_kind || !launches_per_kind) return fail(GPE_ERR_INVALID, "null argument");
+  CHK(set_device(s));
+  s->timer.reset();
+  s->timer.own = true;
+  s->timing = true;
+  const int r = do_step(s, nsteps, t_mid);
+  s->timing = false;
+  const hipError_t e = s->timer.collect(s->st);
+  s->timer.own = false;
+  CHK(r);
+  if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("kernel timing: ") + hipGetErrorString(e));
+  for (int k = 0; k < kPassKinds; ++k) {
+    us_per_kind[k] = s->timer.ms[k] * 1e3;
+    launches_per_kind[k] = s->timer.cnt[k];
+  }
+  return GPE_OK;
+}
+
 int gpe_norm_history(gpe_solve* s, double* out, int64_t count) {
   if (!s || (!out && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
   if (count > s->hist_len) return fail(GPE_ERR_STATE, "fewer steps recorded than requested");
@@ -1225,6 +1294,7 @@ int gpe_norm(gpe_solve* s, double* out) {
 
 int gpe_normalise(gpe_solve* s) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  s->work_is_spectrum = false;
   CHK(set_device(s));
   CHK(norm_any(s));
   s->pending = true;
@@ -1359,13 +1429,14 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
     if (!std::equal(s->var, s->var + kPassKinds, b->solves[0]->var))
       return fail(GPE_ERR_INVALID, "batch members must use the same pass variants");
   for (gpe_solve* s : b->solves) {
+    s->work_is_spectrum = false;          // the batch advances psi
     if (b->kind == GPE_IMAG) {
       CHK(ensure_hist(s, nsteps));
       CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
       if (!s->pending) {
-        // a unit pending scale lets every member take the prescaled path
-        const double one = 1.0;
-        HIPCHK(hipMemcpyAsync(s->scal + 1, &one, sizeof(double), hipMemcpyHostToDevice, s->st));
+        // a unit pending scale lets every member take the prescaled path; the
+        // source is a static constant, so it outlives the asynchronous copy
+        HIPCHK(hipMemcpyAsync(s->scal + 1, &kUnitScale, sizeof(double), hipMemcpyHostToDevice, s->st));
         s->pending = true;
       }
       *prescale_first = true;
@@ -1386,9 +1457,9 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   PassArgsRaw a{b->desc, sa, sb, flags, hist, 0};
-  hipEvent_t stop = nullptr;
-  if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &stop));
-  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, nullptr, stop);
+  hipEvent_t start = nullptr, stop = nullptr;
+  if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
+  hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, start, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
 }
@@ -1541,6 +1612,63 @@ int gpe_batch_step_timed(gpe_batch* b, int64_t nsteps, double* ms_per_kind, int6
 int gpe_sync(gpe_solve* s) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_transform(gpe_solve* s) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  CHK(set_device(s));
+  CHK(transform_into_work(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
+int gpe_transform_download(gpe_solve* s, double* host, int64_t offset, int64_t count) {
+  if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  if (!s->work_is_spectrum) return fail(GPE_ERR_STATE, "no gpe_transform of the current field");
+  CHK(check_range(s, offset, count));
+  CHK(set_device(s));
+  return download_field(s, s->work, host, offset, count);
+}
+
+int gpe_monitor_images(gpe_solve* s, int32_t which, double* tof_xy, double* tof_yz, double* tof_slice_xy,
+                       double* tof_slice_yz, double* density_xy, double* plane_mid) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (which & ~GPE_MON_ALL) return fail(GPE_ERR_INVALID, "unknown monitor image bit");
+  double* outs[6] = {tof_xy, tof_yz, tof_slice_xy, tof_slice_yz, density_xy, plane_mid};
+  for (int b = 0; b < 6; ++b)
+    if ((which >> b & 1) && !outs[b]) return fail(GPE_ERR_INVALID, "null output for a requested image");
+  if (!which) return GPE_OK;
+  CHK(set_device(s));
+  const long long plane = (long long)s->n * s->n;
+  if (!s->mon) CHK(dmalloc((void**)&s->mon, (size_t)plane * 7 * sizeof(double)));
+  if (which & (GPE_MON_TOF_XY | GPE_MON_TOF_YZ | GPE_MON_TOF_SLICE_XY | GPE_MON_TOF_SLICE_YZ)) {
+    if (!s->work_is_spectrum) CHK(transform_into_work(s));
+  } else {
+    CHK(materialize(s));
+  }
+  const bool c64 = s->prec == GPE_C64;
+  auto sum = [&](const void* f, int axis, double* out) {
+    if (c64) launch_abs2_sum<float>(f, s->n, axis, out, s->st);
+    else launch_abs2_sum<double>(f, s->n, axis, out, s->st);
+  };
+  auto cut = [&](const void* f, int axis, int index, void* out_c128, double* out_abs2) {
+    if (c64) launch_plane<float>(f, s->n, axis, index, out_c128, out_abs2, s->st);
+    else launch_plane<double>(f, s->n, axis, index, out_c128, out_abs2, s->st);
+  };
+  if (which & GPE_MON_TOF_XY) sum(s->work, 2, s->mon);
+  if (which & GPE_MON_TOF_YZ) sum(s->work, 0, s->mon + plane);
+  if (which & GPE_MON_TOF_SLICE_XY) cut(s->work, 2, 0, nullptr, s->mon + 2 * plane);
+  if (which & GPE_MON_TOF_SLICE_YZ) cut(s->work, 0, 0, nullptr, s->mon + 3 * plane);
+  if (which & GPE_MON_DENSITY_XY) sum(s->psi, 2, s->mon + 4 * plane);
+  if (which & GPE_MON_PLANE_MID) cut(s->psi, 2, s->n / 2, s->mon + 5 * plane, nullptr);
+  HIPCHK(hipGetLastError());
+  for (int b = 0; b < 6; ++b) {
+    if (!(which >> b & 1)) continue;
+    const size_t bytes = (size_t)plane * sizeof(double) * (b == 5 ? 2 : 1);
+    HIPCHK(hipMemcpyAsync(outs[b], s->mon + b * plane, bytes, hipMemcpyDeviceToHost, s->st));
+  }
   HIPCHK(hipStreamSynchronize(s->st));
   return GPE_OK;
 }

@@ -447,6 +447,18 @@ __device__ __forceinline__ void phase_sincos(double x, double* s, double* c) {
   quadrant(quad, sn, cs, s, c);
 }
 
+// ------------------------------------------------------ wave-local exchange --
+// Orders one wave's LDS accesses across its lanes without a workgroup barrier.
+// A wave issues its LDS instructions in program order and the LDS serves them
+// in that order, so every lane's writes before this point are seen by every
+// lane's reads after it once the compiler keeps the accesses on their side of
+// it (the wavefront-scope fences; no instruction is emitted for them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------- reductions --
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

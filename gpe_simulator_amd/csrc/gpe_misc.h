@@ -34,6 +34,9 @@ template <typename R> void launch_to_c128(const void* in, void* out, long long c
 template <typename R>
 void launch_norm(const void* psi, long long count, double* red, unsigned int* counter, double* scal, double dx3,
                  int prescale, hipStream_t st);
+template <typename R> void launch_abs2_sum(const void* f, int n, int axis, double* out, hipStream_t st);
+template <typename R>
+void launch_plane(const void* f, int n, int axis, int index, void* out_c128, double* out_abs2, hipStream_t st);
 void launch_finish_sums(const double* red, long long nblk, int nq, double* out, hipStream_t st);
 void launch_finish_nk(const double* occ, const unsigned int* cnt, long long nblk, int nbins, double* out_occ,
                       long long* out_cnt, hipStream_t st);

@@ -295,6 +295,61 @@ __global__ void k_finish_nk(const double* occ, const unsigned int* cnt, long lon
   }
 }
 
+// ------------------------------------------------------ monitor images --
+// Projections of |f|^2 for the monitor images, summed in FP64 in a fixed
+// order (deterministic):
+//   k_abs2_sum_z:  out[i*n + j] = sum_k |f[i][j][k]|^2, one wave per (i, j)
+//                  row (contiguous loads, lanes stride the row, then a wave
+//                  reduction);
+//   k_abs2_sum_x:  out[j*n + k] = sum_i |f[i][j][k]|^2, one thread per (j, k)
+//                  column walking i (adjacent threads read adjacent
+//                  elements of each plane).
+template <typename R>
+__global__ void __launch_bounds__(256) k_abs2_sum_z(const cpx<R>* f, int n, double* out) {
+  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= (long long)n * n) return;
+  const cpx<R>* r = f + row * n;
+  double acc = 0.0;
+  for (int k = lane; k < n; k += 64) {
+    const cpx<R> v = gload(r + k);
+    acc += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) out[row] = acc;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_abs2_sum_x(const cpx<R>* f, int n, double* out) {
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)n * n;
+  if (col >= plane) return;
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const cpx<R> v = gload(f + i * plane + col);
+    acc += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+  }
+  out[col] = acc;
+}
+
+// One plane of f as complex128: axis 2 gives out[i*n + j] = f[i][j][index]
+// (a strided gather), axis 0 gives out[j*n + k] = f[index][j][k].  With
+// `abs2`, |.|^2 in FP64 is written to out_abs2 instead.
+template <typename R>
+__global__ void __launch_bounds__(256) k_plane(const cpx<R>* f, int n, int axis, int index, cpx<double>* out,
+                                               double* out_abs2) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long plane = (long long)n * n;
+  if (p >= plane) return;
+  const long long src = axis == 2 ? p * n + index : (long long)index * plane + p;
+  const cpx<R> v = gload(f + src);
+  if (out_abs2) {
+    out_abs2[p] = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+  } else {
+    out[p] = cpx<double>{double(v.x), double(v.y)};
+  }
+}
+
 // Streaming copy used to calibrate bandwidth counters.
 __global__ void k_copy(const float4* in, float4* out, long long count) {
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < count;
@@ -396,6 +451,21 @@ void launch_copy(const void* in, void* out, long long bytes, hipStream_t st) {
                      count);
 }
 
+template <typename R>
+void launch_abs2_sum(const void* f, int n, int axis, double* out, hipStream_t st) {
+  const long long plane = (long long)n * n;
+  if (axis == 2)
+    hipLaunchKernelGGL(k_abs2_sum_z<R>, dim3((unsigned)((plane + 3) / 4)), dim3(256), 0, st, (const cpx<R>*)f, n,
+                       out);
+  else
+    hipLaunchKernelGGL(k_abs2_sum_x<R>, dim3(nblocks(plane, 256)), dim3(256), 0, st, (const cpx<R>*)f, n, out);
+}
+template <typename R>
+void launch_plane(const void* f, int n, int axis, int index, void* out_c128, double* out_abs2, hipStream_t st) {
+  hipLaunchKernelGGL(k_plane<R>, dim3(nblocks((long long)n * n, 256)), dim3(256), 0, st, (const cpx<R>*)f, n, axis,
+                     index, (cpx<double>*)out_c128, out_abs2);
+}
+
 #define GPE_INST(R)                                                                                            \
   template void launch_twiddle<R>(void*, int, hipStream_t);                                                  \
   template void launch_kinetic<R>(void*, void*, void*, const double*, int, double, double, int, double,     \
@@ -410,7 +480,9 @@ void launch_copy(const void* in, void* out, long long bytes, hipStream_t st) {
   template void launch_from_c128<R>(const void*, void*, long long, hipStream_t);                             \
   template void launch_to_c128<R>(const void*, void*, long long, hipStream_t);                               \
   template void launch_norm<R>(const void*, long long, double*, unsigned int*, double*, double, int,         \
-                               hipStream_t);
+                               hipStream_t);                                                              \
+  template void launch_abs2_sum<R>(const void*, int, int, double*, hipStream_t);                             \
+  template void launch_plane<R>(const void*, int, int, int, void*, double*, hipStream_t);
 GPE_INST(float)
 GPE_INST(double)
 

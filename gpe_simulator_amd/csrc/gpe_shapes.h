@@ -7,7 +7,10 @@
 // reads B*sizeof(complex) contiguous bytes per row; in Z passes a block's tile
 // is B*N contiguous elements.  Each pass kind of a solve can run a different
 // variant (all read and write the same canonical field), and the autotuner
-// picks per kind; variant 0 is the default before tuning.
+// picks per kind; variant 0 is the default before tuning.  All variants of one
+// (precision, side) share TP, hence the codelets, twiddles and order of every
+// sum: they differ only in tiling, so a field's arithmetic, and its result to
+// the last bit, does not depend on which variant the tuner picked.
 //
 // One line per variant: X(precision 0/1 = complex64/complex128, side, variant,
 // TP, B, W, RD), RD the exchange rounds of the Z passes (2: half the block's
@@ -23,7 +26,7 @@
   X(0, 128, 0, 8, 32, 3, 1) X(0, 128, 1, 8, 16, 3, 1) X(0, 128, 2, 8, 16, 4, 1)                           \
   X(0, 192, 0, 8, 16, 3, 1) X(0, 192, 1, 8, 32, 3, 1) X(0, 192, 2, 8, 16, 4, 1) X(0, 192, 3, 8, 16, 3, 2) \
   X(0, 256, 0, 16, 16, 3, 1) X(0, 256, 1, 16, 16, 4, 1) X(0, 256, 2, 16, 32, 3, 1)                        \
-  X(1, 64, 0, 8, 16, 2, 1) X(1, 64, 1, 8, 32, 2, 1) X(1, 64, 2, 4, 16, 2, 1)                              \
+  X(1, 64, 0, 8, 16, 2, 1) X(1, 64, 1, 8, 32, 2, 1)                                                       \
   X(1, 96, 0, 4, 16, 2, 1) X(1, 96, 1, 4, 32, 2, 1) X(1, 96, 2, 4, 8, 2, 1) X(1, 96, 3, 4, 32, 2, 2)      \
   X(1, 128, 0, 8, 16, 2, 1) X(1, 128, 1, 8, 32, 2, 1) X(1, 128, 2, 8, 8, 2, 1)                            \
   X(1, 192, 0, 8, 8, 2, 1) X(1, 192, 1, 8, 16, 2, 1) X(1, 192, 2, 8, 16, 2, 2)                            \

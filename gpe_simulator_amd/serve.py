@@ -6,10 +6,12 @@ resident in HBM and, once per tick:
 
   1. admits waiting requests while free HBM (gpe_mem_info) minus a reserve
      covers the request's resident + diagnostics footprint (gpe_solve_footprint);
-  2. groups the active solves by (grid side, precision, kind) and enqueues
-     `check_every` steps for every group as one batch (one launch per pass
+  2. groups the active solves by (grid side, precision, kind, steps this
+     tick) and enqueues each group's steps as one batch (one launch per pass
      for all members, replayed from a hipGraph), each group on its own HIP
-     stream so groups overlap on the device, then waits for all groups;
+     stream so groups overlap on the device, then waits for all groups.  A
+     tick is `check_every` steps, or fewer for a real-time run's last tick,
+     which stops on its requested step count;
   3. reads each member's per-step norms, decides convergence (imaginary time:
      |dmu| < tol on the norm-decay estimate, with complex64 members moving to
      a complex128 field when their Switch says so) or completion (real
@@ -137,6 +139,9 @@ class Server:
     # -- main loop
     def run(self, requests, realtime=True, log=None):
         """Serve `requests` (arrival-ordered) and return (outcomes, stats)."""
+        for req in requests:
+            if req.kind == "real" and req.steps < 1:
+                raise ValueError(f"request {req.id}: a real-time run needs at least one step")
         start = time.perf_counter()
         incoming = deque(sorted(requests, key=lambda r: r.arrival))
         waiting, active, outcomes = deque(), [], []
@@ -163,31 +168,36 @@ class Server:
                     active.append(self._admit(req, now))
                     free -= need
             self.max_active = max(self.max_active, len(active))
-            # one tick: every group enqueued, then all awaited
+            # one tick: every group enqueued, then all awaited.  A real-time
+            # member steps at most its remaining count, so a run of any length
+            # ends on its requested step; members with equal step counts share
+            # a group.
             groups = {}
             for slot in active:
-                groups.setdefault((slot.req.n, slot.precision, slot.req.kind), []).append(slot)
+                steps = self._tick_steps(slot)
+                groups.setdefault((slot.req.n, slot.precision, slot.req.kind, steps), []).append(slot)
             tick0 = time.perf_counter()
             # every batch is built (device allocation) before any group runs
-            launched = [(members, self._batch_for(members)) for members in groups.values()]
-            for _, batch in launched:
-                batch.launch(self.check_every, graph=self.graph)
-            for _, batch in launched:
+            launched = [(key[3], members, self._batch_for(members)) for key, members in groups.items()]
+            for steps, _, batch in launched:
+                batch.launch(steps, graph=self.graph)
+            for _, _, batch in launched:
                 batch.wait()
             # per-step norms of every imaginary-time member, one fetch per group
             norms = {}
-            for members, batch in launched:
+            for steps, members, batch in launched:
                 if members[0].req.kind == "imaginary":
-                    for slot, row in zip(members, batch.norm_history(self.check_every)):
+                    for slot, row in zip(members, batch.norm_history(steps)):
                         norms[id(slot)] = row
+            stepped = {id(slot): steps for steps, members, _ in launched for slot in members}
             if not realtime:
                 # without sleeping for arrivals, time still passes while the device works
                 virtual += time.perf_counter() - tick0
             # progress, convergence and completion
             finished = []
             for slot in active:
-                slot.steps += self.check_every
-                self.grid_point_steps += slot.req.n ** 3 * self.check_every
+                slot.steps += stepped[id(slot)]
+                self.grid_point_steps += slot.req.n ** 3 * stepped[id(slot)]
                 req = slot.req
                 if req.kind == "imaginary":
                     try:
@@ -221,6 +231,13 @@ class Server:
             batch.close()
         self._batches.clear()
         return outcomes, stats
+
+    def _tick_steps(self, slot):
+        """Steps `slot` takes this tick: check_every, or a real-time run's
+        remaining steps when fewer are left."""
+        if slot.req.kind == "real":
+            return min(self.check_every, slot.req.steps - slot.steps)
+        return self.check_every
 
     def _promote(self, slot):
         """Move a complex64 imaginary-time member to a complex128 field."""

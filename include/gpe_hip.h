@@ -261,6 +261,43 @@ int gpe_step_timed(gpe_solve* solve, int64_t nsteps, const double* t_mid, double
  * (about 1000 launches at 256^3). */
 int gpe_step_stamped(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* us_per_kind,
                      int64_t* launches_per_kind);
+/* fftn(psi), raw and unnormalised as numpy's and CuPy's fftn (the psi_k
+ * of gpe/save.py:31), computed on the device into the solve's work field;
+ * psi is unchanged.  gpe_transform_download copies elements of it to the
+ * host as complex128 (GPE_ERR_STATE unless a gpe_transform of the current
+ * field came before: any step, upload or diagnostics call invalidates it). */
+int gpe_transform(gpe_solve* solve);
+int gpe_transform_download(gpe_solve* solve, double* host_c128, int64_t offset, int64_t count);
+
+/* The monitor images of gpe/save.py:59-90 as raw FP64 sums on the device
+ * (sum_abs2 of gpe/helper.py:3-12), each an n*n C-order plane; the caller
+ * applies the reference's scale factors and fftshift:
+ *   GPE_MON_TOF_XY        sum_z |F|^2, F = fftn(psi)   -> tof_xy[i*n + j]
+ *   GPE_MON_TOF_YZ        sum_x |F|^2                  -> tof_yz[j*n + k]
+ *   GPE_MON_TOF_SLICE_XY  |F[:, :, 0]|^2               -> tof_slice_xy
+ *   GPE_MON_TOF_SLICE_YZ  |F[0, :, :]|^2               -> tof_slice_yz
+ *   GPE_MON_DENSITY_XY    sum_z |psi|^2                -> density_xy
+ *   GPE_MON_PLANE_MID     psi[:, :, n/2] as complex128 -> plane_mid (2*n*n doubles)
+ * `which` ORs the images wanted; outputs of other images may be NULL.  One
+ * transform serves every momentum image (and is reused while the field is
+ * unchanged since gpe_transform). */
+#define GPE_MON_TOF_XY 1
+#define GPE_MON_TOF_YZ 2
+#define GPE_MON_TOF_SLICE_XY 4
+#define GPE_MON_TOF_SLICE_YZ 8
+#define GPE_MON_DENSITY_XY 16
+#define GPE_MON_PLANE_MID 32
+#define GPE_MON_ALL 63
+int gpe_monitor_images(gpe_solve* solve, int32_t which, double* tof_xy, double* tof_yz, double* tof_slice_xy,
+                       double* tof_slice_yz, double* density_xy, double* plane_mid);
+
+/* gpe_step with every pass launch carrying its own start and stop event,
+ * both timestamped by the dispatch itself (hipExtLaunchKernel), and nothing
+ * recorded between launches: a span is the kernel's execution alone, the
+ * duration a kernel trace reports for the same launch.  Reports summed
+ * microseconds per pass kind and launch counts like gpe_step_timed. */
+int gpe_step_kernel_times(gpe_solve* solve, int64_t nsteps, const double* t_mid, double* us_per_kind,
+                          int64_t* launches_per_kind);
 /* Device-to-device streaming copy of `bytes` (multiple of 16), repeated
  * `reps` times; returns milliseconds per copy.  Bandwidth calibration. */
 int gpe_bench_copy(gpe_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy);

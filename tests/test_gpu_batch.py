@@ -191,3 +191,71 @@ def test_server_warm_up_tunes_every_shape(require_gpu):
     again = server.warm(shapes)
     assert again < seconds
     ctx.close()
+
+
+def test_server_real_run_stops_on_its_step_count(require_gpu):
+    """A real-time request whose step count is not a multiple of the tick
+    (120 steps, ticks of 50) takes a short last tick: it reports 120 steps
+    and the diagnostics of the state after exactly 120 steps."""
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.grids import get_grids
+    from gpe_simulator_amd.serve import Request, Server
+    req = Request(id=0, arrival=0.0, n=64, kind="real", precision="c128", beta=300.0, dt=5e-4, steps=120)
+    outcomes, _ = Server(check_every=50).run([req])
+    assert outcomes[0].steps == 120
+    x, k = get_grids(req.n, req.length)
+    solve = make_solve(req.n, x, k, "real", req.dt, req.beta, static=req.static, precision="c128")
+    solve.gaussian(req.sigma)
+    solve.step(120)
+    diag = solve.diagnostics(120 * req.dt, spectrum=False)
+    solve.close()
+    assert outcomes[0].mu == pytest.approx(diag["mu"], rel=1e-12)
+    assert outcomes[0].energy == pytest.approx(diag["E"], rel=1e-12)
+    with pytest.raises(ValueError):
+        Server(check_every=50).run([Request(id=1, arrival=0.0, n=64, kind="real", precision="c64",
+                                            beta=300.0, steps=0)])
+
+
+def test_config_sweep_64_solves_at_96(require_gpu):
+    """The batched parameter sweep at its full size: 64 imaginary-time
+    ground states at 96^3 on [-10,10)^3 with seed-drawn g, omega_z and start
+    widths (seed 0), relaxed to |dmu| < 2e-9 (reference units): every member
+    converges, and a fixed sample of members matches the same request solved
+    alone in complex128 to 1e-8 in mu and E."""
+    from gpe_simulator_amd.sweep import config_sweep, solve_sequential, sweep_batched
+    reqs = config_sweep(count=64, seed=0)
+    assert len(reqs) == 64 and {r.n for r in reqs} == {96}
+    results = sweep_batched(reqs)
+    assert all(r.converged for r in results)
+    for i in (0, 21, 42, 63):
+        seq = solve_sequential(reqs[i])
+        assert seq.converged
+        assert results[i].mu == pytest.approx(seq.mu, rel=1e-8)
+        assert results[i].energy == pytest.approx(seq.energy, rel=1e-8)
+
+
+def test_serving_stream_every_side_kind_and_precision(require_gpu):
+    """The serving stream at its full request mix: a seeded Poisson stream of
+    12 requests holding every side of {64, 96, 128, 192, 256}, ground states
+    and 2000-step real-time runs, complex64 and complex128.  Requests overlap
+    on the device, and every result is the same as the request served alone
+    (same step count; mu and E to 1e-12)."""
+    from gpe_simulator_amd.serve import Server, request_stream
+    reqs = request_stream(12, rate_per_s=8.0, seed=2)
+    assert {r.n for r in reqs} == {64, 96, 128, 192, 256}
+    assert {(r.kind, r.precision) for r in reqs} == {(k, p) for k in ("imaginary", "real") for p in ("c64", "c128")}
+    assert {r.steps for r in reqs if r.kind == "real"} == {2000}
+    mixed, stats = Server(check_every=50).run(reqs)
+    assert stats["requests"] == len(reqs)
+    assert stats["max_active"] >= 3
+    assert stats["p50_ms"] <= stats["p99_ms"]
+    by_id = {o.id: o for o in mixed}
+    for req in reqs:
+        alone, _ = Server(check_every=50).run([req])
+        a, m = alone[0], by_id[req.id]
+        assert m.converged and a.converged, req
+        assert m.steps == a.steps
+        if req.kind == "real":
+            assert m.steps == req.steps
+        assert m.mu == pytest.approx(a.mu, rel=1e-12)
+        assert m.energy == pytest.approx(a.energy, rel=1e-12)

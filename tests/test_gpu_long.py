@@ -26,6 +26,32 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+def threaded_run(p, psi, steps, workers=16):
+    """Problem.run of the oracle at large sizes: the same per-point functions
+    (real_half_step, the kinetic factor exp(-i k^2 dt)) applied slab by slab
+    on worker threads, and the oracle's threaded 3D FFTs.  Real time, no
+    momentum cutoff."""
+    from concurrent.futures import ThreadPoolExecutor
+    assert p.kind == "real" and p.kcut is None
+    psi = np.array(psi, dtype=np.complex128, copy=True)
+    kin = np.exp(-1j * orc.k_squared(p.k) * p.dt)
+    slabs = [slice(i, min(i + 8, p.n)) for i in range(0, p.n, 8)]
+    with ThreadPoolExecutor(workers) as pool:
+        def each(fn):
+            list(pool.map(fn, slabs))
+        for i in range(steps):
+            potential = p.U
+            if p.dynamic_source:
+                potential = potential + p.W(i * p.dt + p.dt / 2)
+            half = lambda s: orc.real_half_step(psi[s], p.v_static[s], potential[s], p.beta, p.dt)  # noqa: E731
+            each(half)
+            psi_k = orc._fftn(psi)
+            each(lambda s: psi_k[s].__imul__(kin[s]))
+            psi[...] = orc._ifftn(psi_k)
+            each(half)
+    return psi
+
+
 def engine(problem, precision, kind=None):
     from gpe_simulator_amd.engine import make_solve
     return make_solve(problem.n, problem.x, problem.k, kind or problem.kind, problem.dt, problem.beta,
@@ -177,3 +203,49 @@ def test_quench_128_c64_against_c128(require_gpu):
     assert b1["mu"] == pytest.approx(a1["mu"], rel=1e-6)
     rho128, rho64 = np.abs(psi128) ** 2, np.abs(psi64) ** 2
     assert np.linalg.norm(rho64 - rho128) / np.linalg.norm(rho128) < 1e-4
+
+
+def test_quench_trap_1000_steps_c128_at_128(require_gpu):
+    """Config 2's grid and trap at full size: 128^3 on [-12,12)^3,
+    V = x^2 + y^2 + 16 z^2, beta = 1100 (the quenched g = 550), dt = 5e-5,
+    1000 real-time steps from a seeded Thomas-Fermi start: complex128 psi
+    within relative L2 1e-10 of the FP64 oracle, mu and E within 1e-11."""
+    p = orc.Problem(128, 24.0, "real", 5e-5, 1100.0, static="x**2 + y**2 + 16*z**2")
+    psi0 = orc.synthetic_state(128, 24.0, seed=21, beta=1100.0, potential=p.U)
+    expected = threaded_run(p, psi0, 1000)
+    solve = engine(p, "c128")
+    solve.upload(psi0)
+    solve.step(1000)
+    assert rel_l2(solve.to_numpy(), expected) < 1e-10
+    e_ref, e_dev = p.energies(expected), solve.diagnostics(1000 * p.dt, spectrum=False)
+    assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-11)
+    assert e_dev["E"] == pytest.approx(e_ref["E"], rel=1e-11)
+    assert e_dev["norm"] == pytest.approx(orc.get_N(psi0, p.x), rel=1e-12)
+    solve.close()
+
+
+def test_stirring_50_steps_c128_at_256(require_gpu):
+    """The headline workload's own parameters at full size: 256^3 on
+    [-20,20)^3, V = r^2, beta = 4000, the moving beam, dt = 2.5e-4, 50 steps
+    from the seeded Thomas-Fermi start: complex128 psi within relative L2
+    1e-10 of the FP64 oracle, mu and E within 1e-11, and the complex64 run
+    of the same steps within 1e-6 of the oracle's mu."""
+    from gpe_simulator_amd.engine import midpoint_times
+    p = orc.Problem(256, 40.0, "real", 2.5e-4, 4000.0, static="x**2 + y**2 + z**2", dynamic=BEAM)
+    psi0 = orc.synthetic_state(256, 40.0, seed=0, beta=4000.0)
+    expected = threaded_run(p, psi0, 50)
+    t = 50 * p.dt
+    e_ref = p.energies(expected, t)
+    for precision in ("c128", "c64"):
+        solve = engine(p, precision)
+        solve.upload(psi0)
+        solve.step(50, midpoint_times(0, 50, p.dt))
+        e_dev = solve.diagnostics(t, spectrum=False)
+        if precision == "c128":
+            assert rel_l2(solve.to_numpy(), expected) < 1e-10
+            assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-11)
+            assert e_dev["E"] == pytest.approx(e_ref["E"], rel=1e-11)
+        else:
+            assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-6)
+            assert rel_l2(solve.to_numpy(), expected) < 1e-4
+        solve.close()

@@ -360,3 +360,40 @@ def test_stamped_step_matches_the_plain_step_and_reports_every_launch(require_gp
             assert 0 < us / count < 5e3, (kind, us, count)
     stamped.close()
     plain.close()
+
+
+@pytest.mark.parametrize("kind", ["real", "imaginary"])
+def test_kernel_timed_step_matches_the_plain_step(kind, require_gpu):
+    """gpe_step_kernel_times advances the state exactly as gpe_step and
+    reports one positive duration per pass launch, each no longer than the
+    stream interval around the whole call."""
+    from gpe_simulator_amd.engine import midpoint_times
+    n, length, dt, steps = 64, 12.0, 1e-3, 6
+    static = "x**2 + y**2 + z**2"
+    beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)" if kind == "real" else None
+    _, timed = make(n, length, kind, dt, 30.0, static, beam)
+    _, plain = make(n, length, kind, dt, 30.0, static, beam)
+    psi0 = random_state(n, length, seed=5)
+    tm = midpoint_times(0, steps, dt) if beam else None
+    for solve in (timed, plain):
+        solve.upload(psi0)
+    timed.mark(0)
+    spans = timed.step_kernel_times(steps, tm)
+    timed.mark(1)
+    whole_us = timed.marked_ms() * 1e3
+    plain.step(steps, tm)
+    np.testing.assert_array_equal(timed.to_numpy(), plain.to_numpy())
+    counts = {k: v[1] for k, v in spans.items() if v[1]}
+    if kind == "real":
+        assert counts == {"z_first": 1, "z_mid": steps - 1, "z_last": 1, "y_fwd": steps, "y_inv": steps,
+                          "x_kin": steps}
+    else:
+        assert counts == {"z_first": steps, "z_last": steps, "y_fwd": steps, "y_inv": steps, "x_kin": steps}
+    total = 0.0
+    for name, (us, count) in spans.items():
+        if count:
+            assert 0 < us / count < 5e3, (name, us, count)
+            total += us
+    assert total <= whole_us
+    timed.close()
+    plain.close()

@@ -91,3 +91,19 @@ def test_headline_kernels_use_no_scratch(tmp_path):
         for name, scratch, vgpr in kernel_metadata(obj, tmp_path):
             if "k_pass" in name:
                 assert scratch == 0, f"{obj.name}: {name} uses {scratch} B/lane of scratch ({vgpr} VGPRs)"
+
+
+def test_variants_of_one_shape_share_their_arithmetic():
+    """Every compiled variant of one (precision, side) has the same threads per
+    pencil, so the same codelets, twiddles and summation order: the tuner's
+    choice of tile cannot change a result, and a batch member stays
+    bit-identical to the same solve stepped alone whatever each was tuned to."""
+    text = (ROOT / "gpe_simulator_amd" / "csrc" / "gpe_shapes.h").read_text()
+    block = text[text.index("#define GPE_VARIANTS(X)"):text.index("namespace gpe")]
+    tps = {}
+    for p, n, _v, tp, _b, _w, _rd in re.findall(
+            r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", block):
+        tps.setdefault((int(p), int(n)), set()).add(int(tp))
+    assert tps
+    mixed = {shape: sorted(t) for shape, t in tps.items() if len(t) > 1}
+    assert not mixed, f"shapes with variants of different threads per pencil: {mixed}"
