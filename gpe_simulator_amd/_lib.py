@@ -35,8 +35,12 @@ EXPORTS = (
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
-    "gpe_step_kernel_times",
+    "gpe_step_kernel_times", "gpe_transform", "gpe_transform_download", "gpe_monitor_images",
+    "gpe_volume_download",
 )
+
+# gpe_monitor_images bits (GPE_MON_* in include/gpe_hip.h), in output order
+MONITOR_IMAGES = ("TOF_xy", "TOF_yz", "TOF_slice_xy", "TOF_slice_yz", "density_xy", "plane_mid")
 
 
 class GpeError(RuntimeError):
@@ -96,6 +100,10 @@ def _declare(lib):
         "gpe_step_timed": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_step_stamped": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_step_kernel_times": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
+        "gpe_transform": (c_int, [vp]),
+        "gpe_volume_download": (c_int, [vp, i32, dp, i64, i64]),
+        "gpe_transform_download": (c_int, [vp, dp, i64, i64]),
+        "gpe_monitor_images": (c_int, [vp, i32, dp, dp, dp, dp, dp, dp]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
         "gpe_angular_momentum": (c_int, [vp, dp]),
         "gpe_batch_create": (c_int, [vp, POINTER(vp), i32, POINTER(vp)]),
@@ -403,6 +411,36 @@ class Solve:
         out = c_double()
         check(lib().gpe_angular_momentum(self.handle, byref(out)))
         return out.value
+
+    def volume(self, target):
+        """The FP64 static volume (TGT_VOL real part, TGT_VOL_IM imaginary
+        part) as an (n, n, n) host array."""
+        out = np.empty(self.n ** 3)
+        check(lib().gpe_volume_download(self.handle, int(target), dptr(out), 0, out.size))
+        return out.reshape((self.n,) * 3)
+
+    def spectrum(self):
+        """fftn(psi), raw and unnormalised, computed on the device and copied
+        to the host as a complex128 cube."""
+        check(lib().gpe_transform(self.handle))
+        out = np.empty(self.n ** 3, dtype=np.complex128)
+        check(lib().gpe_transform_download(self.handle, dptr(out.view(np.float64)), 0, out.size))
+        return out.reshape((self.n,) * 3)
+
+    def monitor_images(self, names):
+        """Raw device sums of the monitor images named in `names` (a subset of
+        MONITOR_IMAGES): {name: (n, n) array}; plane_mid is complex128."""
+        which, outs = 0, {}
+        for bit, name in enumerate(MONITOR_IMAGES):
+            if name in names:
+                which |= 1 << bit
+                outs[name] = np.zeros((self.n, self.n), dtype=np.complex128 if name == "plane_mid" else np.float64)
+        unknown = set(names) - set(MONITOR_IMAGES)
+        if unknown:
+            raise ValueError(f"unknown monitor images {sorted(unknown)}")
+        ptrs = [dptr(outs[name].view(np.float64)) if name in outs else None for name in MONITOR_IMAGES]
+        check(lib().gpe_monitor_images(self.handle, which, *ptrs))
+        return outs
 
     def normalise(self):
         check(lib().gpe_normalise(self.handle))

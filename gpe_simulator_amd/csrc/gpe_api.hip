@@ -1175,6 +1175,18 @@ int gpe_volume_upload(gpe_solve* s, int32_t target, const double* host, int64_t 
   return recast(s, s->m_vol[comp], s->w_vol[comp], s->n3);
 }
 
+int gpe_volume_download(gpe_solve* s, int32_t target, double* host, int64_t offset, int64_t count) {
+  if (!s || (!host && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
+  if (target != GPE_TGT_VOL && target != GPE_TGT_VOL_IM) return fail(GPE_ERR_INVALID, "bad volume target");
+  CHK(check_range(s, offset, count));
+  const int comp = target == GPE_TGT_VOL ? 0 : 1;
+  if (!s->m_vol[comp]) return fail(GPE_ERR_STATE, "the solve has no such static volume");
+  CHK(set_device(s));
+  HIPCHK(hipMemcpyAsync(host, s->m_vol[comp] + offset, count * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return GPE_OK;
+}
+
 int gpe_step(gpe_solve* s, int64_t nsteps, const double* t_mid) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   return do_step(s, nsteps, t_mid);

@@ -34,15 +34,26 @@ def shell_summary(counts, occupation, x_array, k_array, n):
     return dict(k=k, nk=nk, Nk=Nk, counts=counts, edges=edges, complete_shells=complete_shells)
 
 
-def nk_extraction(psi, x_array, k_array, t=0.0):
+def _solve_of(psi_k):
+    from .helper import DeviceSpectrum
+    if isinstance(psi_k, DeviceSpectrum):
+        return psi_k.field.solve
+    if isinstance(psi_k, DeviceField):
+        return psi_k.solve
+    raise TypeError("nk_extraction takes fftn(psi) of the engine's field (gpe_simulator_amd.helper.fftn) "
+                    "or the DeviceField itself; the wavefunction stays on the device")
+
+
+def nk_extraction(psi_k, x_array, k_array, slab_size=16, t=0.0):
     """Shell averages of |fftn(psi)|^2; Nk sums to the wavefunction norm.
 
-    `psi` is the real-space DeviceField (the reference passes fftn(psi); here
-    the transform is fused into the histogram passes)."""
-    if not isinstance(psi, DeviceField):
-        raise TypeError("psi must be a DeviceField")
-    diag = psi.solve.diagnostics(t, spectrum=True)
-    return shell_summary(diag["counts"], diag["occupation"], x_array, k_array, psi.solve.n)
+    `psi_k` is fftn(psi) as the reference passes it (a DeviceSpectrum from
+    helper.fftn), or the real-space DeviceField: the transform is fused into
+    the histogram passes either way, and slab_size is accepted for the
+    reference's signature."""
+    solve = _solve_of(psi_k)
+    diag = solve.diagnostics(t, spectrum=True)
+    return shell_summary(diag["counts"], diag["occupation"], x_array, k_array, solve.n)
 
 
 def view_summary(counts, occupation, x_array, k_array, n):
@@ -59,9 +70,8 @@ def view_summary(counts, occupation, x_array, k_array, n):
                 k_max=k_max, nk=Nk / (np.maximum(counts, 1) * dk ** 3), Nk=Nk, counts=counts)
 
 
-def nk_view(psi, x_array, k_array, t=0.0):
+def nk_view(psi_k, x_array, k_array, t=0.0):
     """The 128-bin equal-width view of |fftn(psi)|^2 (see view_summary)."""
-    if not isinstance(psi, DeviceField):
-        raise TypeError("psi must be a DeviceField")
-    diag = psi.solve.diagnostics(t, spectrum=True)
-    return view_summary(diag["view_counts"], diag["view_occupation"], x_array, k_array, psi.solve.n)
+    solve = _solve_of(psi_k)
+    diag = solve.diagnostics(t, spectrum=True)
+    return view_summary(diag["view_counts"], diag["view_occupation"], x_array, k_array, solve.n)

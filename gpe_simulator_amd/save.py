@@ -2,9 +2,11 @@
 
 Same schedule, files and columns as the reference.  Norm, kinetic energy, the
 |psi(k)|^2 shell histogram and Lz come from the engine's FP64 device
-reductions (one forward transform per due step, as the reference shares one
-fftn).  The monitor images (TOF and density projections, slices, phase) are
-small host-side post-processing of a downloaded copy on due steps only.
+reductions.  The monitor images come from device sums as well: one fftn of
+the field on the device serves every momentum image (the reference shares one
+fftn, gpe/save.py:31), and the projections and slices are FP64 sums over the
+device fields (sum_abs2, gpe/helper.py:3-12), so only n x n planes cross to the
+host, where the reference's scale factors, fftshift and the phase mask apply.
 """
 
 from __future__ import annotations
@@ -13,13 +15,14 @@ import csv
 
 import numpy as np
 
-from .field import asnumpy
 from .helper import is_due
 from .nk_extraction import shell_summary, view_summary
 from .state_io import save_wavefunction
 
-TOF_FIELDS = ("show_TOF_image_interval", "show_verticle_TOF_interval", "show_TOF_slice_interval",
-              "show_verticle_TOF_slice_interval")
+# monitor schedule key -> device image (gpe_monitor_images)
+_IMAGE_OF = {"show_TOF_image_interval": "TOF_xy", "show_verticle_TOF_interval": "TOF_yz",
+             "show_TOF_slice_interval": "TOF_slice_xy", "show_verticle_TOF_slice_interval": "TOF_slice_yz",
+             "show_magnifier_interval": "density_xy"}
 
 
 def save_outputs(stage, step, *, initial=False, final=False):
@@ -68,33 +71,31 @@ def save_outputs(stage, step, *, initial=False, final=False):
             save_npz("nkview", **view_summary(diag["view_counts"], diag["view_occupation"], stage.x_array,
                                               stage.k_array, stage.N_grids))
 
-    images = any(due[name] for name in TOF_FIELDS) or due["show_magnifier_interval"] \
-        or due["show_magnifier_slice_interval"] or due["show_phase_interval"]
-    if not images:
-        return
-    host = asnumpy(psi)
-    x, k = np.asarray(stage.x_array), np.asarray(stage.k_array)
-    if any(due[name] for name in TOF_FIELDS):
-        psi_k = np.fft.fftn(host)
-        dx = float(x[1] - x[0])
-        dk = abs(float(k[1] - k[0]))
-        scale = dx ** 3 / (host.size * dk ** 3)
-        ks = np.fft.fftshift(k)
-        power = np.abs(psi_k) ** 2
-        if due["show_TOF_image_interval"]:
-            save_npz("TOF_xy", values=np.fft.fftshift(power.sum(axis=2)) * (scale * dk), kx=ks, ky=ks)
-        if due["show_verticle_TOF_interval"]:
-            save_npz("TOF_yz", values=np.fft.fftshift(power.sum(axis=0)) * (scale * dk), ky=ks, kz=ks)
-        if due["show_TOF_slice_interval"]:
-            save_npz("TOF_slice_xy", values=np.fft.fftshift(power[:, :, 0]) * scale, kx=ks, ky=ks)
-        if due["show_verticle_TOF_slice_interval"]:
-            save_npz("TOF_slice_yz", values=np.fft.fftshift(power[0, :, :]) * scale, ky=ks, kz=ks)
-    if due["show_magnifier_interval"]:
-        save_npz("density_xy", values=(np.abs(host) ** 2).sum(axis=2) * float(x[1] - x[0]), x=x, y=x)
+    wanted = [name for field, name in _IMAGE_OF.items() if due[field]]
     if due["show_magnifier_slice_interval"] or due["show_phase_interval"]:
+        wanted.append("plane_mid")
+    if not wanted:
+        return
+    images = psi.solve.monitor_images(wanted)
+    x, k = np.asarray(stage.x_array), np.asarray(stage.k_array)
+    dx = float(x[1] - x[0])
+    dk = abs(float(k[1] - k[0]))
+    scale = dx ** 3 / (stage.N_grids ** 3 * dk ** 3)
+    ks = np.fft.fftshift(k)
+    if "TOF_xy" in images:
+        save_npz("TOF_xy", values=np.fft.fftshift(images["TOF_xy"]) * (scale * dk), kx=ks, ky=ks)
+    if "TOF_yz" in images:
+        save_npz("TOF_yz", values=np.fft.fftshift(images["TOF_yz"]) * (scale * dk), ky=ks, kz=ks)
+    if "TOF_slice_xy" in images:
+        save_npz("TOF_slice_xy", values=np.fft.fftshift(images["TOF_slice_xy"]) * scale, kx=ks, ky=ks)
+    if "TOF_slice_yz" in images:
+        save_npz("TOF_slice_yz", values=np.fft.fftshift(images["TOF_slice_yz"]) * scale, ky=ks, kz=ks)
+    if "density_xy" in images:
+        save_npz("density_xy", values=images["density_xy"] * dx, x=x, y=x)
+    if "plane_mid" in images:
         middle = stage.N_grids // 2
-        plane = host[:, :, middle]
-        density = np.abs(plane) ** 2
+        plane = images["plane_mid"]
+        density = plane.real ** 2 + plane.imag ** 2
         if due["show_magnifier_slice_interval"]:
             save_npz("density_slice_xy", values=density, x=x, y=x, z=x[middle])
         if due["show_phase_interval"]:

@@ -142,6 +142,10 @@ int gpe_potential_dynamic(gpe_solve* solve, int32_t target, const int32_t* code,
 int gpe_volume_fill(gpe_solve* solve, int32_t target, double value);
 int gpe_volume_upload(gpe_solve* solve, int32_t target, const double* host, int64_t offset, int64_t count,
                       int32_t accumulate);
+/* Copies elements of the FP64 static volume (GPE_TGT_VOL: real part;
+ * GPE_TGT_VOL_IM: imaginary part, the negated absorber of
+ * gpe/potentials.py:31-37) to the host; GPE_ERR_STATE when the solve has none. */
+int gpe_volume_download(gpe_solve* solve, int32_t target, double* host, int64_t offset, int64_t count);
 
 /* Advance nsteps Strang steps.  t_mid holds the FP64 midpoint time of each
  * step (t_i + dt/2, as gpe/make_propagator.py:94); it may be NULL when no

@@ -9,6 +9,8 @@ from unittest.mock import patch
 import numpy as np
 import pytest
 
+from oracle import gpe_oracle as orc
+
 pytestmark = pytest.mark.gpu
 
 TWO_PI = 2 * np.pi
@@ -238,3 +240,80 @@ def test_schema3_nk_view_output(env):
             assert data["edges"][-1] == pytest.approx(float(data["k_max"]), rel=1e-15)
     assert not list(folders[1].glob("nkview_*.npz"))
     assert len(list(folders[1].glob("nk_*.npz"))) == 2
+
+
+@pytest.mark.parametrize("n", [18, 64])
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_device_monitor_images_and_spectrum_match_the_oracle(n, precision, require_gpu):
+    """gpe_monitor_images' raw device sums, turned into the reference's arrays
+    the way save.py does, against the oracle's restatement of
+    gpe/save.py:59-90; the device fftn against numpy's; on the generic (18)
+    and a fast (64) transform, with a pending renormalisation in imaginary
+    time."""
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.field import DeviceField
+    from gpe_simulator_amd.helper import fftn, get_E
+    from gpe_simulator_amd.nk_extraction import nk_extraction
+    length = 12.0
+    x, k = orc.get_grids(n, length)
+    solve = make_solve(n, x, k, "imaginary", 1e-3, 50.0, static="x**2 + y**2 + z**2", precision=precision)
+    rng = np.random.default_rng(n)
+    psi0 = orc.gaussian(x, 1.3) * (1 + 0.05 * rng.standard_normal((n,) * 3)) * np.exp(0.3j * x[None, :, None])
+    solve.upload(psi0)
+    solve.step(3)                                   # leaves a pending 1/sqrt(norm) on the device
+    psi = solve.to_numpy()
+    tol = 1e-12 if precision == "c128" else 3e-6
+    spec = solve.spectrum()
+    np.testing.assert_allclose(spec, np.fft.fftn(psi), rtol=0, atol=tol * np.abs(spec).max())
+    names = ("TOF_xy", "TOF_yz", "TOF_slice_xy", "TOF_slice_yz", "density_xy", "plane_mid")
+    raw = solve.monitor_images(names)
+    dx, dk = x[1] - x[0], abs(k[1] - k[0])
+    scale = dx ** 3 / (n ** 3 * dk ** 3)
+    shift = np.fft.fftshift
+    got = {"TOF_xy": shift(raw["TOF_xy"]) * scale * dk, "TOF_yz": shift(raw["TOF_yz"]) * scale * dk,
+           "TOF_slice_xy": shift(raw["TOF_slice_xy"]) * scale, "TOF_slice_yz": shift(raw["TOF_slice_yz"]) * scale,
+           "density_xy": raw["density_xy"] * dx, "density_slice_xy": np.abs(raw["plane_mid"]) ** 2}
+    ref = orc.monitor_images(psi, x, k)
+    for name, values in got.items():
+        np.testing.assert_allclose(values, ref[name], rtol=0, atol=10 * tol * np.abs(ref[name]).max(), err_msg=name)
+    np.testing.assert_array_equal(raw["plane_mid"], psi[:, :, n // 2])
+    # the reference's helper signatures: get_E and nk_extraction take fftn(psi)
+    field = DeviceField(solve)
+    psi_k = fftn(field)
+    assert get_E(psi_k, x, k) == pytest.approx(orc.get_E(np.fft.fftn(psi), x, k), rel=10 * tol)
+    shells = nk_extraction(psi_k, x, k)
+    np.testing.assert_array_equal(shells["counts"], orc.nk_extraction(np.fft.fftn(psi), x, k)["counts"])
+    with pytest.raises(TypeError):
+        get_E(np.fft.fftn(psi), x, k)
+    solve.close()
+
+
+@pytest.mark.parametrize("n", [18, 64])
+@pytest.mark.parametrize("kind", ["real", "imaginary"])
+def test_box_and_absorber_volumes_match_the_oracle(n, kind, require_gpu):
+    """The static volumes that add_box / add_absorber evaluate on the device
+    (FP64 programs with a max operation) against the oracle's restatement of
+    gpe/potentials.py:3-37: the box as the real part, the absorber as the
+    negated imaginary part, and no imaginary part in imaginary time
+    (gpe/stage.py:57-58)."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.potentials import add_absorber, add_box
+    length = 16.0
+    x, k = orc.get_grids(n, length)
+    box = {"R": 5.0, "L": 6.0, "wall_thickness": 1.0, "trap_depth": 40.0}
+    absorber = {"activation_distance": 2.0, "strength": 3.0}
+    solve = make_solve(n, x, k, kind, 1e-3, 10.0, precision="c64")
+    add_box(solve, box)
+    want_re = orc.box_potential(x, box["R"], box["L"], box["wall_thickness"], box["trap_depth"])
+    np.testing.assert_allclose(solve.volume(_lib.TGT_VOL), want_re, rtol=1e-14, atol=1e-300)
+    if kind == "real":
+        add_absorber(solve, absorber, length)
+        ramps = orc.absorption_potential(absorber["activation_distance"], absorber["strength"], x, length)
+        want = orc.total_static_potential(want_re, ramps)
+        np.testing.assert_allclose(solve.volume(_lib.TGT_VOL_IM), want.imag, rtol=1e-14, atol=1e-14)
+        assert solve.volume(_lib.TGT_VOL_IM).min() == pytest.approx(-3 * 2.0 * 3.0)   # a corner: three faces
+    else:
+        with pytest.raises(_lib.GpeError):
+            solve.volume(_lib.TGT_VOL_IM)
+    solve.close()

@@ -51,6 +51,14 @@ class HostSolve:
     def angular_momentum(self):
         return 2.0
 
+    def monitor_images(self, names):
+        """The raw sums gpe_monitor_images returns: unshifted, unscaled."""
+        power = np.abs(np.fft.fftn(self.psi)) ** 2
+        raw = {"TOF_xy": power.sum(axis=2), "TOF_yz": power.sum(axis=0), "TOF_slice_xy": power[:, :, 0],
+               "TOF_slice_yz": power[0, :, :], "density_xy": (np.abs(self.psi) ** 2).sum(axis=2),
+               "plane_mid": self.psi[:, :, self.n // 2].copy()}
+        return {name: raw[name] for name in names}
+
 
 def workflow_stage(tmp_path, n=4, **save):
     from gpe_simulator_amd.configuration import dump_yaml, new_config
@@ -140,3 +148,19 @@ def test_two_point_grid_has_positive_momentum_density(tmp_path):
     save.save_outputs(item, 1)
     with np.load(item.monitor_dir / "TOF_slice_xy_00000001.npz") as saved:
         assert saved["values"].sum() > 0
+
+
+def test_every_monitor_image_matches_the_oracle(tmp_path):
+    """Each saved monitor array equals the oracle's restatement of
+    gpe/save.py:59-90 (scale factors, fftshift, the mid plane and the phase
+    mask) for the same field."""
+    from gpe_simulator_amd import save
+    item, solve = workflow_stage(tmp_path, n=6)
+    solve.psi[1, 2, 3] = 0
+    item.monitor_params.update({key: 1 for key in item.monitor_params
+                                if key.startswith("show_") and key != "show_nk_interval"})
+    save.save_outputs(item, 1)
+    expected = orc.monitor_images(solve.psi, np.asarray(item.x_array), np.asarray(item.k_array))
+    for name, ref in expected.items():
+        with np.load(item.monitor_dir / f"{name}_00000001.npz") as saved:
+            np.testing.assert_allclose(saved["values"], ref, rtol=1e-12, atol=0)
