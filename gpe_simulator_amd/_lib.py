@@ -36,7 +36,7 @@ EXPORTS = (
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
     "gpe_step_kernel_times", "gpe_transform", "gpe_transform_download", "gpe_monitor_images",
-    "gpe_volume_download",
+    "gpe_volume_download", "gpe_step_noalloc",
 )
 
 # gpe_monitor_images bits (GPE_MON_* in include/gpe_hip.h), in output order
@@ -102,6 +102,7 @@ def _declare(lib):
         "gpe_step_kernel_times": (c_int, [vp, i64, dp, dp, POINTER(i64)]),
         "gpe_transform": (c_int, [vp]),
         "gpe_volume_download": (c_int, [vp, i32, dp, i64, i64]),
+        "gpe_step_noalloc": (c_int, [vp, i64, dp]),
         "gpe_transform_download": (c_int, [vp, dp, i64, i64]),
         "gpe_monitor_images": (c_int, [vp, i32, dp, dp, dp, dp, dp, dp]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
@@ -341,6 +342,16 @@ class Solve:
             if tm.size < nsteps:
                 raise ValueError("t_mid needs one time per step")
         check(lib().gpe_step_unfused(self.handle, int(nsteps), dptr(tm) if tm is not None else None))
+
+    def step_noalloc(self, nsteps=1, t_mid=None):
+        """gpe_step_noalloc: gpe_step without any device allocation (imaginary
+        time keeps its norm partials in the library's static device array)."""
+        tm = None
+        if t_mid is not None:
+            tm = np.ascontiguousarray(t_mid, dtype=np.float64)
+            if tm.size < nsteps:
+                raise ValueError("t_mid needs one time per step")
+        check(lib().gpe_step_noalloc(self.handle, int(nsteps), dptr(tm) if tm is not None else None))
 
     def step_timed(self, nsteps, t_mid=None):
         tm = None if t_mid is None else np.ascontiguousarray(t_mid, dtype=np.float64)

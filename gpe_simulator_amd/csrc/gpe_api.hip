@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -55,6 +56,7 @@ struct PassArgsRaw {
   int flags;
   int hist_index;
   int stamp_index;
+  double* red;
 };
 constexpr long long kMaxStamps = 1LL << 22;  // wall-clock slots of one stamped call (32 MiB)
 static_assert(sizeof(PassArgsRaw) == sizeof(PassArgs<float>), "pass argument layout");
@@ -180,8 +182,26 @@ struct PassTimer {
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free scratch --
+// Device memory of gpe_step_noalloc: the FP64 norm partials of an
+// imaginary-time step (one per Z_LAST block: n^2 / B for a fast side, n^2 for
+// the generic transform), so that the call allocates nothing.  Each GPU the
+// library runs on holds one copy, at its own address, until the process ends.
+// In host code `scratch` names a host-side stand-in, never the device memory:
+// its per-device address comes from hipGetSymbolAddress once per device (at
+// gpe_ctx_create, with the device current) and is the only handle used.
+constexpr long long kScratchDoubles = 1LL << 16;   // 512 KiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};   // per-device address of `scratch`
+}  // namespace
+
 struct gpe_ctx {
   int device = 0;
+  double* scratch = nullptr;              // this device's `scratch` (device address)
   // per-pass-kind variant defaults for new solves, keyed by (side, precision, kind)
   std::map<std::tuple<int, int, int>, std::array<int, kPassKinds>> variants;
 };
@@ -252,6 +272,7 @@ struct gpe_solve {
   void* desc_diag = nullptr;
   std::vector<char> h_desc_step, h_desc_diag;
 
+  double* red_override = nullptr;         // norm partials of the allocation-free step (the device's scratch)
   bool pending = false;                   // psi_true = psi * scal[1]
   bool work_is_spectrum = false;          // the work field holds fftn(psi) of the current psi
   bool timing = false;
@@ -407,7 +428,7 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     // reference time
     s->stamp_chained = false;
   }
-  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index};
+  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
   hipError_t e = s->launch(kind, &a, s->n, 1, s->st, start, stop);
@@ -802,9 +823,22 @@ int gpe_ctx_create(int device, gpe_ctx** out) {
   int count = 0;
   HIPCHK(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return fail(GPE_ERR_INVALID, "no HIP device " + std::to_string(device));
+  if (device >= kMaxDevices) return fail(GPE_ERR_INVALID, "device index beyond the library's table");
   HIPCHK(hipSetDevice(device));
+  double* scratch_ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return fail(GPE_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+      g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    scratch_ptr = g_scratch_ptr[device];
+  }
   gpe_ctx* c = new gpe_ctx();
   c->device = device;
+  c->scratch = scratch_ptr;
   *out = c;
   return GPE_OK;
 }
@@ -977,6 +1011,9 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
   if (hipGetLastError() != hipSuccess) return bail(fail(GPE_ERR_HIP, "table kernels failed to launch"));
   if ((r = ensure_red(s, std::max<long long>(kNormBlocks, s->blocks(K_Z_LAST, n))))) return bail(r);
+  // imaginary time: the per-step norm history exists from the start, so that
+  // gpe_step_noalloc can record it without allocating
+  if (s->kind == GPE_IMAG && (r = ensure_hist(s, 1024))) return bail(r);
   if ((r = refresh_desc(s))) return bail(r);
   *out = s;
   return GPE_OK;
@@ -1190,6 +1227,34 @@ int gpe_volume_download(gpe_solve* s, int32_t target, double* host, int64_t offs
 int gpe_step(gpe_solve* s, int64_t nsteps, const double* t_mid) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   return do_step(s, nsteps, t_mid);
+}
+
+int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (nsteps < 0) return fail(GPE_ERR_INVALID, "nsteps must be >= 0");
+  if (nsteps == 0) return GPE_OK;
+  if (any_dynamic(s) && !t_mid)
+    return fail(GPE_ERR_INVALID, "t_mid is required: a time-dependent potential is registered");
+  if (!s->ctx->scratch) return fail(GPE_ERR_STATE, "the context holds no scratch address");
+  CHK(set_device(s));
+  if (s->kind == GPE_REAL) return step_real(s, nsteps, t_mid);   // real time uses no scratch
+  const long long parts = s->blocks(K_Z_LAST, s->n);
+  if (parts > kScratchDoubles)
+    return fail(GPE_ERR_INVALID, "the allocation-free step holds " + std::to_string(kScratchDoubles) +
+                                     " norm partials; this solve needs " + std::to_string(parts));
+  if (nsteps > s->hist_cap)
+    return fail(GPE_ERR_INVALID, "nsteps " + std::to_string(nsteps) + " exceeds the norm history the solve holds (" +
+                                     std::to_string(s->hist_cap) + " steps); the allocation-free step cannot grow it");
+  const bool pre = s->pending;
+  s->red_override = s->ctx->scratch;
+  const int r = emit_static_steps(GPE_IMAG, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+    return pass(s, k, false, sa, sb, f, h);
+  });
+  s->red_override = nullptr;
+  CHK(r);
+  s->pending = true;
+  s->hist_len = nsteps;
+  return GPE_OK;
 }
 
 int gpe_step_unfused(gpe_solve* s, int64_t nsteps, const double* t_mid) {
@@ -1468,7 +1533,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 }
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
-  PassArgsRaw a{b->desc, sa, sb, flags, hist, 0};
+  PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
   if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
   hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, start, stop);

@@ -99,6 +99,7 @@ struct PassArgs {
   int flags;
   int hist_index;          // step index into s->hist (Z_LAST with F_NORM)
   int stamp_index;         // first of this launch's per-block end slots in s->stamps (F_STAMP)
+  double* red;             // norm partials of Z_LAST / k_norm_finish; null: the solve's own (s->red)
 };
 
 // ------------------------------------------------------------ transforms --
@@ -734,7 +735,8 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       __shared__ double red_scratch[16];
       double part[1] = {acc[0]};
       block_sum<1>(part, red_scratch);
-      if (threadIdx.x == 0) s.red[bid] = part[0];
+      double* red = a.red ? a.red : s.red;
+      if (threadIdx.x == 0) red[bid] = part[0];
     }
   } else if constexpr (KIND == K_Z_DIAG) {
     __shared__ double red_scratch[48];
@@ -775,7 +777,8 @@ __global__ void __launch_bounds__(256) k_norm_finish(PassArgs<R> a, int nblk) {
   const SolveDev<R>& s = a.s[blockIdx.x];
   __shared__ double red_scratch[16];
   double tot[1] = {0.0};
-  for (int q = threadIdx.x; q < nblk; q += blockDim.x) tot[0] += s.red[q];
+  const double* red = a.red ? a.red : s.red;
+  for (int q = threadIdx.x; q < nblk; q += blockDim.x) tot[0] += red[q];
   block_sum<1>(tot, red_scratch);
   if (threadIdx.x == 0) {
     const double norm = s.dx3 * tot[0];

@@ -154,6 +154,21 @@ int gpe_volume_download(gpe_solve* solve, int32_t target, double* host, int64_t 
  * Reference: the propagator closures (gpe/make_propagator.py:12-54, 57-103,
  * 109-151) called once per step by launch_stage (gpe/launch.py:8-12). */
 int gpe_step(gpe_solve* solve, int64_t nsteps, const double* t_mid);
+/* gpe_step that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same kernels, same
+ * results as gpe_step (bit-identical).  In imaginary time the FP64 norm
+ * partials live in the library's file-scope device array (one per GPU, held
+ * until the process ends) instead of the solve's buffer, and the per-step
+ * norm history goes to the solve's existing history buffer, which is never
+ * grown.  Limits (GPE_ERR_INVALID beyond them): at most 65536 norm partials
+ * (every fast side up to 1024, the generic transform up to side 256), and
+ * nsteps at most the solve's history capacity (1024 steps from creation, more
+ * after a gpe_step of more steps).  Real time uses no scratch.  Calls that
+ * overlap on one GPU (solves on different streams or threads) share the
+ * array and race: serialise them. */
+int gpe_step_noalloc(gpe_solve* solve, int64_t nsteps, const double* t_mid);
+
 /* The same real-time steps with the reference's launch structure: one
  * launch per reference kernel and transform axis (half-step, FFT z, y, x,
  * kinetic factor, inverse FFT x, y, z, half-step; gpe/make_propagator.py:38-101
