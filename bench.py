@@ -46,7 +46,13 @@ STATIC = "x**2 + y**2 + z**2"
 BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
 NK_INTERVAL = 500
 HBM_PEAK_GBS = 8000.0
-PASSES_PER_STEP = 4          # steady-state psi sweeps per real-time step (fused z pass)
+# steady-state psi sweeps per real-time step: the complex64 run steps with the
+# four-sweep plan (3D FFT, kinetic factor, inverse 3D FFT in passes z, y, x, y
+# with the half-steps fused into z); the separable plan (one pass per axis,
+# each FFT -> 1D kinetic factor -> inverse FFT) needs 3 but rounds three
+# rotations per point and step, so complex64 real time keeps four
+PASSES_PER_STEP = 4
+PASSES_SEPARABLE = 3
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_stirring256_c64.txt"
 STATS_FILE = ROOT / "profiles" / "kernel_stats_stirring256.md"
 
@@ -68,11 +74,19 @@ def synthetic_state(x, seed=0):
     return psi
 
 
-def run_engine(precision, psi0, x, k, steps, warmup, unfused=False):
+def run_engine(precision, psi0, x, k, steps, warmup, unfused=False, plan=None):
     """Timed steps of the workload; `unfused` runs the reference-structured
-    step (one launch per reference kernel and axis) instead of the fused plan."""
+    step (one launch per reference kernel and axis) instead of the fused
+    plan; `plan` overrides the solve's default step plan."""
     from gpe_simulator_amd.engine import make_solve, midpoint_times
+    from gpe_simulator_amd import tune
     solve = make_solve(N, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=precision)
+    if plan is not None and plan != solve.plan:
+        tune.ensure_tuned(N, solve.precision, solve.kind, plan=plan)
+        fresh = make_solve(N, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=precision)
+        solve.close()
+        solve = fresh
+        solve.set_plan(plan)
     solve.upload(psi0)
     tm = midpoint_times(0, warmup + steps, DT)
     step = solve.step_unfused if unfused else solve.step
@@ -328,14 +342,23 @@ def main(argv=None):
     result_c128, acc = None, None
     c128_steps = args.c128_steps or args.steps
     if not args.skip_c128:
-        log(f"c128 run: {c128_steps} steps")
-        solve128, el128, diag128 = run_engine("c128", psi0, x, k, c128_steps, args.warmup)
+        from gpe_simulator_amd import _lib
+        log(f"c128 run: {c128_steps} steps, the complex64 run's four-sweep plan")
+        solve128, el128, diag128 = run_engine("c128", psi0, x, k, c128_steps, args.warmup,
+                                              plan=_lib.PLAN_FOUR_SWEEP)
         v128 = N ** 3 * c128_steps / el128
-        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / c128_steps,
+        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / c128_steps, "plan": "four-sweep",
                        "speedup_c64_over_c128": value / v128, "mu": diag128["mu"]}
         if c128_steps == args.steps:
             acc = accuracy(psi0, x, final64, (solve128.to_numpy(), diag128))
         solve128.close()
+        log(f"c128 run: {c128_steps} steps, separable plan (the complex128 default)")
+        solve_s, el_s, _ = run_engine("c128", psi0, x, k, c128_steps, args.warmup, plan=_lib.PLAN_SEPARABLE)
+        solve_s.close()
+        vs = N ** 3 * c128_steps / el_s
+        result_c128["separable"] = {"value": vs, "ms_per_step": el_s * 1e3 / c128_steps, "sweeps_per_step": 3,
+                                    "hbm_frac": vs * 3 * 2 * 16 / (HBM_PEAK_GBS * 1e9),
+                                    "speedup_c64_over_it": value / vs}
         unfused_steps = min(c128_steps, args.unfused_steps)
         log(f"reference-structured c128 run: {unfused_steps} steps")
         solve_u, el_u, diag_u = run_engine("c128", psi0, x, k, unfused_steps, args.warmup, unfused=True)
@@ -410,6 +433,7 @@ def main(argv=None):
                      "traced_frac": (bytes_per_launch / (dom_traced_us * 1e-6) / 1e9 / HBM_PEAK_GBS
                                      if dom_traced_us else None)},
         "step_roofline_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
+        "step_bytes_per_point": step_bytes,
         "stream_copy_gbs": copy_gbs,
         "per_kernel": per_kernel,
         "mu_c64": diag64["mu"],

@@ -22,7 +22,19 @@ VIEW_BINS = 128                  # GPE_VIEW_BINS: equal-width bins of the spectr
 REAL, IMAG = 0, 1
 TGT_XY, TGT_Z, TGT_VOL, TGT_VOL_IM = 0, 1, 2, 3
 
-PASS_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin", "z_diag", "x_diag", "z_fwd", "y_lz", "x_lz")
+PASS_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin", "z_diag", "x_diag", "z_fwd", "y_lz", "x_lz",
+              "zk_first", "zk_mid", "yk", "xk", "xk_last")
+# step plans (GPE_PLAN_*): three sweeps per step with the kinetic factor split
+# per axis, or the four-sweep 3D-transform plan (the only one with a cutoff)
+PLAN_SEPARABLE, PLAN_FOUR_SWEEP = 0, 1
+
+
+def default_plan(precision, kind, masked=False):
+    """The plan a new solve steps with (gpe_solve_create's rule): four-sweep
+    with a momentum cutoff and for complex64 real time, else separable."""
+    if masked or (int(precision) == C64 and int(kind) == REAL):
+        return PLAN_FOUR_SWEEP
+    return PLAN_SEPARABLE
 
 EXPORTS = (
     "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",
@@ -36,7 +48,7 @@ EXPORTS = (
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
     "gpe_step_kernel_times", "gpe_transform", "gpe_transform_download", "gpe_monitor_images",
-    "gpe_volume_download", "gpe_step_noalloc",
+    "gpe_volume_download", "gpe_step_noalloc", "gpe_solve_set_plan", "gpe_solve_get_plan",
 )
 
 # gpe_monitor_images bits (GPE_MON_* in include/gpe_hip.h), in output order
@@ -103,6 +115,8 @@ def _declare(lib):
         "gpe_transform": (c_int, [vp]),
         "gpe_volume_download": (c_int, [vp, i32, dp, i64, i64]),
         "gpe_step_noalloc": (c_int, [vp, i64, dp]),
+        "gpe_solve_set_plan": (c_int, [vp, i32]),
+        "gpe_solve_get_plan": (c_int, [vp, POINTER(i32)]),
         "gpe_transform_download": (c_int, [vp, dp, i64, i64]),
         "gpe_monitor_images": (c_int, [vp, i32, dp, dp, dp, dp, dp, dp]),
         "gpe_bench_copy": (c_int, [vp, i64, i32, dp]),
@@ -169,7 +183,8 @@ class Context:
         check(lib().gpe_ctx_create(int(device), byref(handle)))
         self.handle = handle
         self.device = device
-        self.variant_defaults = {}     # (side, precision, kind) -> variants set on this context
+        self.variant_defaults = {}     # (side, precision, kind) -> variants a caller set on this context
+        self.tuned_defaults = {}       # (side, precision, kind) -> variants the autotuner set
         self.tunings = {}              # autotuner results cached on this context
 
     def mem_info(self):
@@ -177,11 +192,22 @@ class Context:
         check(lib().gpe_mem_info(self.handle, byref(free), byref(total)))
         return free.value, total.value
 
-    def set_variants(self, n, precision, kind, variants):
-        """Pass variants (one per kind in PASS_KINDS) for solves created later."""
+    def set_variants(self, n, precision, kind, variants, tuned=False):
+        """Pass variants (one per kind in PASS_KINDS) for solves created later.
+        The autotuner passes tuned=True; a caller's own choice (tuned=False)
+        is never replaced by tuning."""
         v = np.ascontiguousarray(variants, dtype=np.int32)
         check(lib().gpe_ctx_set_variants(self.handle, int(n), int(precision), int(kind), i32ptr(v), v.size))
-        self.variant_defaults[(int(n), int(precision), int(kind))] = [int(x) for x in v]
+        key = (int(n), int(precision), int(kind))
+        if tuned:
+            self.tuned_defaults[key] = [int(x) for x in v]
+        else:
+            self.variant_defaults[key] = [int(x) for x in v]
+
+    def default_variants(self, n, precision, kind):
+        """Variants new solves of (side, precision, kind) start from."""
+        key = (int(n), int(precision), int(kind))
+        return self.variant_defaults.get(key) or self.tuned_defaults.get(key) or [0] * len(PASS_KINDS)
 
     def device_info(self):
         """{"name", "runtime_version", "driver_version"} of the context's device."""
@@ -380,6 +406,16 @@ class Solve:
         check(lib().gpe_step_kernel_times(self.handle, int(nsteps), dptr(tm) if tm is not None else None,
                                           dptr(us), i64ptr(cnt)))
         return {k: (float(us[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def set_plan(self, plan):
+        """Step plan: PLAN_SEPARABLE (default) or PLAN_FOUR_SWEEP."""
+        check(lib().gpe_solve_set_plan(self.handle, int(plan)))
+
+    @property
+    def plan(self):
+        out = c_int32()
+        check(lib().gpe_solve_get_plan(self.handle, byref(out)))
+        return out.value
 
     def set_variants(self, variants):
         """Pass variant per kind (sequence in PASS_KINDS order, or a dict)."""

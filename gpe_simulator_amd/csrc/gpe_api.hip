@@ -38,7 +38,7 @@ int fail(int code, const std::string& msg) {
     if (r_ != GPE_OK) return r_;   \
   } while (0)
 
-constexpr int kPassKinds = 11;
+constexpr int kPassKinds = 16;
 constexpr long long kDynBudget = 64LL << 20;   // bytes of time-dependent slot tables
 constexpr long long kStageElems = 1LL << 20;   // complex128 elements per staging copy
 const double kUnitScale = 1.0;                  // host source of unit pending scales
@@ -234,6 +234,8 @@ struct gpe_solve {
   void* work = nullptr;
   void* tw = nullptr;
   void* kin[3] = {nullptr, nullptr, nullptr};
+  void* kin1 = nullptr;                   // separable plan: 1D factor by |frequency|, 1/n folded in
+  int plan = GPE_PLAN_SEPARABLE;          // step plan (GPE_PLAN_*)
   double* xa = nullptr;
   double* ka = nullptr;
 
@@ -335,6 +337,7 @@ void fill_desc(gpe_solve* s, SolveDev<R>& d, bool diag) {
   d.kin_x = (const cpx<double>*)s->kin[0];
   d.kin_y = (const cpx<double>*)s->kin[1];
   d.kin_z = (const cpx<double>*)s->kin[2];
+  d.kin1 = (const cpx<double>*)s->kin1;
   d.karr = s->ka;
   d.xarr = s->xa;
   if (s->dyn[0].active) {
@@ -591,64 +594,91 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream
   return GPE_OK;
 }
 
-// Pass sequence of `nsteps` steps without time-dependent programs, emitted
-// through pass_fn(kind, slot_a, slot_b, flags, hist_index).  Real time fuses
-// the second half-step of step i with the first of step i+1 (Z_MID); imaginary
-// time ends every step with the FP64 norm and defers the 1/sqrt(norm) scale to
-// the next load (F_PRESCALE).
+// Pass sequence of steps [q0, q0 + count) of a call of `nsteps` steps,
+// emitted through pass_fn(kind, slot_a, slot_b, flags, hist_index).  With
+// time-dependent tables (`dyn`) slot q - q0 holds step q's midpoint time, so
+// a chunk of `count` steps reads count + 1 slots; otherwise every slot is 0.
+//
+// Separable plan (GPE_PLAN_SEPARABLE), 3 sweeps per step:
+//   real:      [ZK_FIRST(0)] YK  XK ZK_MID(q, q+1)  YK  XK ...  YK XK_LAST(n-1)
+//   imaginary: ZK_FIRST(prescale) YK XK_LAST(norm)     per step
+// Four-sweep plan (GPE_PLAN_FOUR_SWEEP, and masked solves):
+//   real:      [Z_FIRST(0)] Y_FWD X_KIN Y_INV Z_MID(q, q+1) ... Z_LAST(n-1)
+//   imaginary: Z_FIRST(prescale) Y_FWD X_KIN Y_INV Z_LAST(norm)
+// Real time fuses the closing half-step of step q with the opening one of
+// step q+1; imaginary time ends every step with the FP64 norm and defers the
+// 1/sqrt(norm) scale to the next load (F_PRESCALE).
 template <class P>
-int emit_static_steps(int kind, int64_t nsteps, bool prescale_first, P&& pass_fn) {
-  for (int64_t step = 0; step < nsteps; ++step) {
-    const int first_flags = (step == 0 ? prescale_first : true) ? F_PRESCALE : 0;
+int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bool prescale_first, bool dyn,
+               P&& pass_fn) {
+  const bool sep = plan == GPE_PLAN_SEPARABLE;
+  for (int64_t q = q0; q < q0 + count; ++q) {
+    const int sa = dyn ? (int)(q - q0) : 0, sb = dyn ? sa + 1 : 0;
+    const bool first = q == 0, last = q == nsteps - 1;
     if (kind == GPE_REAL) {
-      if (step == 0) CHK(pass_fn(K_Z_FIRST, 0, 0, prescale_first ? F_PRESCALE : 0, 0));
-      CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
-      CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
-      CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
-      if (step == nsteps - 1)
-        CHK(pass_fn(K_Z_LAST, 0, 0, 0, 0));
-      else
-        CHK(pass_fn(K_Z_MID, 0, 0, 0, 0));
+      const int pre = first && prescale_first ? F_PRESCALE : 0;
+      if (sep) {
+        if (first) CHK(pass_fn(K_ZK_FIRST, sa, 0, pre, 0));
+        CHK(pass_fn(K_YK, 0, 0, 0, 0));
+        if (last) {
+          CHK(pass_fn(K_XK_LAST, sa, 0, 0, 0));
+        } else {
+          CHK(pass_fn(K_XK, 0, 0, 0, 0));
+          CHK(pass_fn(K_ZK_MID, sa, sb, 0, 0));
+        }
+      } else {
+        if (first) CHK(pass_fn(K_Z_FIRST, sa, 0, pre, 0));
+        CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
+        CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
+        CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
+        if (last) CHK(pass_fn(K_Z_LAST, sa, 0, 0, 0));
+        else CHK(pass_fn(K_Z_MID, sa, sb, 0, 0));
+      }
     } else {
-      CHK(pass_fn(K_Z_FIRST, 0, 0, first_flags, 0));
-      CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
-      CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
-      CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
-      CHK(pass_fn(K_Z_LAST, 0, 0, F_NORM, (int)step));
+      const int pre = (first ? prescale_first : true) ? F_PRESCALE : 0;
+      if (sep) {
+        CHK(pass_fn(K_ZK_FIRST, 0, 0, pre, 0));
+        CHK(pass_fn(K_YK, 0, 0, 0, 0));
+        CHK(pass_fn(K_XK_LAST, 0, 0, F_NORM, (int)q));
+      } else {
+        CHK(pass_fn(K_Z_FIRST, 0, 0, pre, 0));
+        CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
+        CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
+        CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
+        CHK(pass_fn(K_Z_LAST, 0, 0, F_NORM, (int)q));
+      }
     }
   }
   return GPE_OK;
 }
 
+// Default step plan.  The momentum cutoff is not separable.  complex64 real
+// time keeps the four-sweep plan: the separable plan rounds three small
+// rotations per point and step (one per axis) instead of one, and each
+// rounding of a rotation by a tiny angle leans outward (the larger component
+// stays put while the smaller one moves), so the norm drifts; measured over
+// the 10000-step 128^3 quench at dt = 5e-5: +2.5e-5 against -1.2e-6 with one
+// rotation.  Imaginary time (a real decay, renormalised every step) and
+// complex128 (rounding at 1e-16) take the separable plan.
+int default_plan(int prec, int kind, int masked) {
+  if (masked || (prec == GPE_C64 && kind == GPE_REAL)) return GPE_PLAN_FOUR_SWEEP;
+  return GPE_PLAN_SEPARABLE;
+}
+
+// The pass kind that writes an imaginary-time step's norm partials.
+int norm_kind(int plan) { return plan == GPE_PLAN_SEPARABLE ? K_XK_LAST : K_Z_LAST; }
+
 int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
-  const bool dyn = any_dynamic(s);
-  if (!dyn) {
-    const bool pre = s->pending;
-    s->pending = false;
-    return emit_static_steps(GPE_REAL, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
-      return pass(s, k, false, sa, sb, f, h);
-    });
-  }
+  const bool pre = s->pending;
+  s->pending = false;
+  auto fn = [&](int k, int sa, int sb, int f, int h) { return pass(s, k, false, sa, sb, f, h); };
+  if (!any_dynamic(s)) return emit_steps(GPE_REAL, s->plan, 0, nsteps, nsteps, pre, false, fn);
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t cs = std::min<int64_t>(s->chunk, nsteps - done);
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     CHK(eval_dynamic(s, tmid + done, nslots, false));
-    for (int64_t q = 0; q < cs; ++q) {
-      const int64_t step = done + q;
-      const int sa = (int)q, sb = (int)q + 1;
-      if (step == 0) {
-        CHK(pass(s, K_Z_FIRST, false, sa, 0, s->pending ? F_PRESCALE : 0));
-        s->pending = false;
-      }
-      CHK(pass(s, K_Y_FWD, false, 0, 0, 0));
-      CHK(pass(s, K_X_KIN, false, 0, 0, 0));
-      CHK(pass(s, K_Y_INV, false, 0, 0, 0));
-      if (step == nsteps - 1)
-        CHK(pass(s, K_Z_LAST, false, sa, 0, 0));
-      else
-        CHK(pass(s, K_Z_MID, false, sa, sb, 0));
-    }
+    CHK(emit_steps(GPE_REAL, s->plan, done, cs, nsteps, pre, true, fn));
     done += cs;
   }
   return GPE_OK;
@@ -706,9 +736,9 @@ int ensure_hist(gpe_solve* s, int64_t nsteps) {
 
 int step_imag(gpe_solve* s, int64_t nsteps) {
   CHK(ensure_hist(s, nsteps));
-  CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
+  CHK(ensure_red(s, s->blocks(norm_kind(s->plan), s->n)));
   const bool pre = s->pending;
-  CHK(emit_static_steps(GPE_IMAG, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+  CHK(emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   }));
   s->pending = true;
@@ -919,6 +949,23 @@ int gpe_solve_get_variants(gpe_solve* s, int32_t* variants, int32_t nkinds) {
   return GPE_OK;
 }
 
+int gpe_solve_set_plan(gpe_solve* s, int32_t plan) {
+  if (!s) return fail(GPE_ERR_INVALID, "null solve");
+  if (plan != GPE_PLAN_SEPARABLE && plan != GPE_PLAN_FOUR_SWEEP) return fail(GPE_ERR_INVALID, "unknown step plan");
+  if (plan == GPE_PLAN_SEPARABLE && s->masked)
+    return fail(GPE_ERR_INVALID, "the momentum cutoff is not separable: a masked solve steps with the four-sweep plan");
+  CHK(set_device(s));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->plan = plan;
+  return ensure_red(s, s->blocks(norm_kind(plan), s->n));
+}
+
+int gpe_solve_get_plan(gpe_solve* s, int32_t* plan) {
+  if (!s || !plan) return fail(GPE_ERR_INVALID, "null argument");
+  *plan = s->plan;
+  return GPE_OK;
+}
+
 int gpe_ctx_set_variants(gpe_ctx* ctx, int32_t n, int32_t precision, int32_t kind, const int32_t* variants,
                          int32_t nkinds) {
   if (!ctx || !variants) return fail(GPE_ERR_INVALID, "null argument");
@@ -994,7 +1041,8 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   const int n = s->n;
   if ((r = dzalloc(&s->psi, s->n3 * s->esz, s->st)) || (r = dzalloc(&s->tw, n * 16, s->st)) ||
       (r = dzalloc(&s->kin[0], n * 16, s->st)) || (r = dzalloc(&s->kin[1], n * 16, s->st)) ||
-      (r = dzalloc(&s->kin[2], n * 16, s->st)) || (r = dzalloc(&s->xa, n * sizeof(double), s->st)) ||
+      (r = dzalloc(&s->kin[2], n * 16, s->st)) || (r = dzalloc(&s->kin1, (n / 2 + 1) * 16, s->st)) ||
+      (r = dzalloc(&s->xa, n * sizeof(double), s->st)) ||
       (r = dzalloc(&s->ka, n * sizeof(double), s->st)) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double), s->st)) ||
       (r = dzalloc(&s->m_z, n * sizeof(double), s->st)) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz, s->st)) ||
       (r = dzalloc(&s->w_z, n * s->rsz, s->st)) || (r = dzalloc(&s->counter, 64, s->st)) ||
@@ -1008,9 +1056,11 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   const double inv_n3 = 1.0 / double(s->n3);
   const int imag = s->kind == GPE_IMAG ? 1 : 0;
   launch_twiddle<double>(s->tw, n, s->st);
-  launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
+  launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->kin1, s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
+  s->plan = default_plan(s->prec, s->kind, s->masked);
   if (hipGetLastError() != hipSuccess) return bail(fail(GPE_ERR_HIP, "table kernels failed to launch"));
-  if ((r = ensure_red(s, std::max<long long>(kNormBlocks, s->blocks(K_Z_LAST, n))))) return bail(r);
+  if ((r = ensure_red(s, std::max<long long>(kNormBlocks, std::max(s->blocks(K_Z_LAST, n), s->blocks(K_XK_LAST, n))))))
+    return bail(r);
   // imaginary time: the per-step norm history exists from the start, so that
   // gpe_step_noalloc can record it without allocating
   if (s->kind == GPE_IMAG && (r = ensure_hist(s, 1024))) return bail(r);
@@ -1023,7 +1073,7 @@ int gpe_solve_destroy(gpe_solve* s) {
   if (!s) return GPE_OK;
   (void)hipSetDevice(s->ctx->device);
   if (s->st) (void)hipStreamSynchronize(s->st);
-  void* ptrs[] = {s->psi, s->work, s->tw, s->kin[0], s->kin[1], s->kin[2], s->xa, s->ka, s->m_xy, s->m_z,
+  void* ptrs[] = {s->psi, s->work, s->tw, s->kin[0], s->kin[1], s->kin[2], s->kin1, s->xa, s->ka, s->m_xy, s->m_z,
                   s->m_vol[0], s->m_vol[1], s->w_xy, s->w_z, s->w_vol[0], s->w_vol[1], s->d_tab[0], s->d_tab[1],
                   s->d_tab[2], s->g_tab[0], s->g_tab[1], s->g_tab[2], s->d_tlist, s->red, s->counter, s->scal,
                   s->hist, s->nkocc, s->nkcnt, s->d_out, s->d_nkocc, s->d_nkcnt, s->stage, s->desc_step,
@@ -1238,7 +1288,7 @@ int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
   if (!s->ctx->scratch) return fail(GPE_ERR_STATE, "the context holds no scratch address");
   CHK(set_device(s));
   if (s->kind == GPE_REAL) return step_real(s, nsteps, t_mid);   // real time uses no scratch
-  const long long parts = s->blocks(K_Z_LAST, s->n);
+  const long long parts = s->blocks(norm_kind(s->plan), s->n);
   if (parts > kScratchDoubles)
     return fail(GPE_ERR_INVALID, "the allocation-free step holds " + std::to_string(kScratchDoubles) +
                                      " norm partials; this solve needs " + std::to_string(parts));
@@ -1247,7 +1297,7 @@ int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
                                      std::to_string(s->hist_cap) + " steps); the allocation-free step cannot grow it");
   const bool pre = s->pending;
   s->red_override = s->ctx->scratch;
-  const int r = emit_static_steps(GPE_IMAG, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+  const int r = emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   });
   s->red_override = nullptr;
@@ -1448,6 +1498,7 @@ struct gpe_batch {
   hipGraphExec_t exec = nullptr;
   int64_t graph_steps = 0;
   int graph_var[kPassKinds] = {0};      // pass variants the graph was captured with
+  int graph_plan = -1;                  // step plan the graph was captured with
   std::vector<char> uploaded;           // descriptor bytes last copied to `desc`
   bool timing = false;
   PassTimer timer;
@@ -1505,11 +1556,13 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
   for (gpe_solve* s : b->solves)
     if (!std::equal(s->var, s->var + kPassKinds, b->solves[0]->var))
       return fail(GPE_ERR_INVALID, "batch members must use the same pass variants");
+  for (gpe_solve* s : b->solves)
+    if (s->plan != b->solves[0]->plan) return fail(GPE_ERR_INVALID, "batch members must use the same step plan");
   for (gpe_solve* s : b->solves) {
     s->work_is_spectrum = false;          // the batch advances psi
     if (b->kind == GPE_IMAG) {
       CHK(ensure_hist(s, nsteps));
-      CHK(ensure_red(s, s->blocks(K_Z_LAST, s->n)));
+      CHK(ensure_red(s, s->blocks(norm_kind(s->plan), s->n)));
       if (!s->pending) {
         // a unit pending scale lets every member take the prescaled path; the
         // source is a static constant, so it outlives the asynchronous copy
@@ -1565,7 +1618,7 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   bool pre = false;
   CHK(batch_prepare(b, nsteps, &pre));
   auto emit = [&]() {
-    return emit_static_steps(b->kind, nsteps, pre, [&](int k, int sa, int sb, int f, int h) {
+    return emit_steps(b->kind, b->solves[0]->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
       return batch_pass(b, k, sa, sb, f, h);
     });
   };
@@ -1573,7 +1626,8 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
     CHK(emit());
   } else {
     const int* var = b->solves[0]->var;
-    if (!b->exec || b->graph_steps != nsteps || !std::equal(var, var + kPassKinds, b->graph_var)) {
+    if (!b->exec || b->graph_steps != nsteps || !std::equal(var, var + kPassKinds, b->graph_var) ||
+        b->graph_plan != b->solves[0]->plan) {
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
@@ -1590,6 +1644,7 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
       b->graph = g;
       HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->graph_steps = nsteps;
+      b->graph_plan = b->solves[0]->plan;
       std::copy(var, var + kPassKinds, b->graph_var);
     }
     HIPCHK(hipGraphLaunch(b->exec, b->st));
@@ -1644,24 +1699,14 @@ int gpe_batch_step_times(gpe_batch* b, int64_t nsteps, const double* t_mid, int6
   // Same pass sequence as a single real-time solve (step_real); before each
   // chunk every time-dependent member evaluates its own slots on the batch
   // stream, so the tables are ordered with the passes that read them.
+  auto fn = [&](int k, int sa, int sb, int f, int h) { return batch_pass(b, k, sa, sb, f, h); };
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t cs = std::min<int64_t>(chunk, nsteps - done);
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     for (size_t i = 0; i < b->solves.size(); ++i)
       if (any_dynamic(b->solves[i])) CHK(eval_dynamic(b->solves[i], t_mid + i * stride + done, nslots, false, b->st));
-    for (int64_t q = 0; q < cs; ++q) {
-      const int64_t step = done + q;
-      const int sa = (int)q, sb = (int)q + 1;
-      if (step == 0) CHK(batch_pass(b, K_Z_FIRST, sa, 0, pre ? F_PRESCALE : 0, 0));
-      CHK(batch_pass(b, K_Y_FWD, 0, 0, 0, 0));
-      CHK(batch_pass(b, K_X_KIN, 0, 0, 0, 0));
-      CHK(batch_pass(b, K_Y_INV, 0, 0, 0, 0));
-      if (step == nsteps - 1)
-        CHK(batch_pass(b, K_Z_LAST, sa, 0, 0, 0));
-      else
-        CHK(batch_pass(b, K_Z_MID, sa, sb, 0, 0));
-    }
+    CHK(emit_steps(GPE_REAL, b->solves[0]->plan, done, cs, nsteps, pre, true, fn));
     done += cs;
   }
   batch_finish(b, nsteps);

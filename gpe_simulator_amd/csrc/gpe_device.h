@@ -106,6 +106,15 @@ __device__ __forceinline__ cpx<double> cmul_c(cpx<double> a, cconst<double> w) {
   return cpx<double>{a.x * w.c - a.y * w.s, a.x * w.s + a.y * w.c};
 }
 
+// a * c for a real factor c held as the real part of a cconst (its hi + lo
+// pair for complex64): the imaginary-time kinetic factor.
+__device__ __forceinline__ cpx<float> scale_re(cpx<float> a, cconst<float> w) {
+  return cpx<float>{fmaf(a.x, w.ch, a.x * w.cl), fmaf(a.y, w.ch, a.y * w.cl)};
+}
+__device__ __forceinline__ cpx<double> scale_re(cpx<double> a, cconst<double> w) {
+  return cpx<double>{a.x * w.c, a.y * w.c};
+}
+
 // A real constant in the same representation (float hi + lo for complex64).
 template <typename R>
 struct rconst {

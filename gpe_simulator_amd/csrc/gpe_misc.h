@@ -13,8 +13,8 @@ constexpr int kNormBlocks = 1024;
 
 template <typename R> void launch_twiddle(void* tw, int n, hipStream_t st);
 template <typename R>
-void launch_kinetic(void* kx, void* ky, void* kz, const double* karr, int n, double dt, double coef, int imag,
-                    double inv_n3, hipStream_t st);
+void launch_kinetic(void* kx, void* ky, void* kz, void* k1, const double* karr, int n, double dt, double coef,
+                    int imag, double inv_n3, hipStream_t st);
 void launch_vm_static(const int* code, int ncode, const double* consts, int target, int n, const double* xa,
                       long long size, double scale, double* master, hipStream_t st);
 template <typename R>

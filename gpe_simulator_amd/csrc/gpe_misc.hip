@@ -34,9 +34,11 @@ __global__ void k_twiddle(cpx<R>* tw, int n) {
 }
 
 // Kinetic factors per axis in FP64: exp(-i*coef*k^2*dt) (real time) or
-// exp(-coef*k^2*dt) (imaginary time).  The x factor also carries 1/n^3.
+// exp(-coef*k^2*dt) (imaginary time).  The x factor also carries 1/n^3.  k1
+// (n/2 + 1 entries, by |frequency|: the k of entry f is karr[f]) is the
+// separable plan's 1D factor, which carries 1/n for its axis's inverse FFT.
 template <typename R>
-__global__ void k_kinetic(cpx<R>* kx, cpx<R>* ky, cpx<R>* kz, const double* karr, int n, double dt,
+__global__ void k_kinetic(cpx<R>* kx, cpx<R>* ky, cpx<R>* kz, cpx<R>* k1, const double* karr, int n, double dt,
                           double coef, int imag, double inv_n3) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -57,6 +59,7 @@ __global__ void k_kinetic(cpx<R>* kx, cpx<R>* ky, cpx<R>* kz, const double* karr
   ky[i] = e;
   kz[i] = e;
   kx[i] = cpx<R>{R(re * inv_n3), R(im * inv_n3)};
+  if (i <= n / 2) k1[i] = cpx<R>{R(re / n), R(im / n)};
 }
 
 // ------------------------------------------------------ expression VM --
@@ -373,10 +376,10 @@ void launch_twiddle(void* tw, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_twiddle<R>, dim3(nblocks(n, 256)), dim3(256), 0, st, (cpx<R>*)tw, n);
 }
 template <typename R>
-void launch_kinetic(void* kx, void* ky, void* kz, const double* karr, int n, double dt, double coef, int imag,
-                    double inv_n3, hipStream_t st) {
+void launch_kinetic(void* kx, void* ky, void* kz, void* k1, const double* karr, int n, double dt, double coef,
+                    int imag, double inv_n3, hipStream_t st) {
   hipLaunchKernelGGL(k_kinetic<R>, dim3(nblocks(n, 256)), dim3(256), 0, st, (cpx<R>*)kx, (cpx<R>*)ky,
-                     (cpx<R>*)kz, karr, n, dt, coef, imag, inv_n3);
+                     (cpx<R>*)kz, (cpx<R>*)k1, karr, n, dt, coef, imag, inv_n3);
 }
 void launch_vm_static(const int* code, int ncode, const double* consts, int target, int n, const double* xa,
                       long long size, double scale, double* master, hipStream_t st) {
@@ -468,8 +471,8 @@ void launch_plane(const void* f, int n, int axis, int index, void* out_c128, dou
 
 #define GPE_INST(R)                                                                                            \
   template void launch_twiddle<R>(void*, int, hipStream_t);                                                  \
-  template void launch_kinetic<R>(void*, void*, void*, const double*, int, double, double, int, double,     \
-                                  hipStream_t);                                                              \
+  template void launch_kinetic<R>(void*, void*, void*, void*, const double*, int, double, double, int,      \
+                                  double, hipStream_t);                                                      \
   template void launch_vm_dynamic<R>(const int*, int, const double*, int, int, const double*, long long,     \
                                      const double*, const double*, int, void*, hipStream_t);                 \
   template void launch_cast<R>(const double*, void*, long long, hipStream_t);                                \

@@ -48,10 +48,23 @@ static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEve
 
 static int axis_of(int kind) {
   switch (kind) {
-    case K_Y_FWD: case K_Y_INV: case K_Y_LZ: return 1;
-    case K_X_KIN: case K_X_DIAG: case K_X_LZ: case K_X_FWD: case K_X_INV: return 0;
+    case K_Y_FWD: case K_Y_INV: case K_Y_LZ: case K_YK: return 1;
+    case K_X_KIN: case K_X_DIAG: case K_X_LZ: case K_X_FWD: case K_X_INV: case K_XK: case K_XK_LAST: return 0;
     default: return 2;
   }
+}
+
+// A pass that writes FP64 norm partials, followed by k_norm_finish (one block
+// per solve) on the same stream; the finisher carries the end event.
+template <int AXIS, int KIND>
+static void go_norm(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEvent_t t0, hipEvent_t t1) {
+  go<AXIS, KIND>(n, batch, st, a, t0, nullptr);
+  const dim3 grid((unsigned)batch), block(256);
+  const int nblk = (int)tile_count<TF, AXIS>(n);
+  if (t1)
+    hipExtLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, nullptr, t1, 0, a, nblk);
+  else
+    hipLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, a, nblk);
 }
 
 // Launches one pass over `batch` solves of side n sharing one descriptor
@@ -65,17 +78,16 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
     case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a, t0, t1); break;
     case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a, t0, t1); break;
     case K_Z_LAST:
-      if (a.flags & F_NORM) {
-        go<2, K_Z_LAST>(n, batch, st, a, t0, nullptr);
-        const dim3 grid((unsigned)batch), block(256);
-        const int nblk = (int)tile_count<TF, 2>(n);
-        if (t1)
-          hipExtLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, nullptr, t1, 0, a, nblk);
-        else
-          hipLaunchKernelGGL((k_norm_finish<Real>), grid, block, 0, st, a, nblk);
-      } else {
-        go<2, K_Z_LAST>(n, batch, st, a, t0, t1);
-      }
+      if (a.flags & F_NORM) go_norm<2, K_Z_LAST>(n, batch, st, a, t0, t1);
+      else go<2, K_Z_LAST>(n, batch, st, a, t0, t1);
+      break;
+    case K_ZK_FIRST: go<2, K_ZK_FIRST>(n, batch, st, a, t0, t1); break;
+    case K_ZK_MID: go<2, K_ZK_MID>(n, batch, st, a, t0, t1); break;
+    case K_YK: go<1, K_YK>(n, batch, st, a, t0, t1); break;
+    case K_XK: go<0, K_XK>(n, batch, st, a, t0, t1); break;
+    case K_XK_LAST:
+      if (a.flags & F_NORM) go_norm<0, K_XK_LAST>(n, batch, st, a, t0, t1);
+      else go<0, K_XK_LAST>(n, batch, st, a, t0, t1);
       break;
     case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a, t0, t1); break;
     case K_Z_FWD: go<2, K_Z_FWD>(n, batch, st, a, t0, t1); break;

@@ -38,12 +38,22 @@ enum PassKind : int {
   K_Z_FWD = 8,     // plain FFT_z (psi -> work)
   K_Y_LZ = 9,      // FFT_y, FP64 sum x * k_y * |F|^2 (no write)
   K_X_LZ = 10,     // FFT_x, FP64 sum y * k_x * |F|^2 (no write)
+  // The separable plan: exp(-i k^2 dt) = exp(-i kx^2 dt) exp(-i ky^2 dt)
+  // exp(-i kz^2 dt), and a factor of one wavenumber commutes with the
+  // transforms along the other axes, so the kinetic step is three 1D
+  // propagators, each an FFT -> factor -> inverse FFT inside one pass: one
+  // sweep of psi per axis and step.
+  K_ZK_FIRST = 11, // [scale] -> half(slot_a) -> FFT_z -> kz factor -> IFFT_z
+  K_ZK_MID = 12,   // half(slot_a) -> half(slot_b) -> FFT_z -> kz factor -> IFFT_z   (real time)
+  K_YK = 13,       // FFT_y -> ky factor -> IFFT_y
+  K_XK = 14,       // FFT_x -> kx factor -> IFFT_x
+  K_XK_LAST = 15,  // FFT_x -> kx factor -> IFFT_x -> half(slot_a) [-> FP64 norm partials]
   // Plain steps of the reference-structured (unfused) real-time step: one
   // launch per reference kernel and transform axis, for measurement.
-  K_Z_HALF = 11,   // half(slot_a) alone, in real space
-  K_Z_INV = 12,    // plain IFFT_z
-  K_X_FWD = 13,    // plain FFT_x
-  K_X_INV = 14,    // plain IFFT_x
+  K_Z_HALF = 16,   // half(slot_a) alone, in real space
+  K_Z_INV = 17,    // plain IFFT_z
+  K_X_FWD = 18,    // plain FFT_x
+  K_X_INV = 19,    // plain IFFT_x
 };
 
 enum PassFlags : int {
@@ -65,6 +75,7 @@ struct SolveDev {
   const cpx<double>* kin_x;  // kinetic factors per axis, FP64 (1/n^3 folded into x)
   const cpx<double>* kin_y;
   const cpx<double>* kin_z;
+  const cpx<double>* kin1;   // 1D factor of one axis by |frequency| (n/2 + 1 entries, 1/n folded in), FP64
   const double* karr;      // k axis, FP64
   const double* xarr;      // x axis, FP64
   const R* pxy;            // potential table over (x, y): [slot][n*n]
@@ -476,6 +487,30 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       pz_lds[e] = pz;
     }
   }
+  // the separable plan's 1D factor, by |frequency| (n/2 + 1 entries)
+  constexpr bool kKin1 =
+      KIND == K_ZK_FIRST || KIND == K_ZK_MID || KIND == K_YK || KIND == K_XK || KIND == K_XK_LAST;
+  // (staged from the FP64 table in the field's constant representation,
+  // float hi + lo for complex64, so the product is one rounding of a factor
+  // carried to ~2^-48 in float arithmetic alone)
+  constexpr bool kStageKin1 = TF::kFast && kKin1;
+  __shared__ cconst<R> kin1_lds[kStageKin1 ? TF::N / 2 + 1 : 1];
+  if constexpr (kStageKin1) {
+    for (int e = threadIdx.x; e <= TF::N / 2; e += TF::THREADS) {
+      const cpx<double> f = gload(s.kin1 + e);
+      kin1_lds[e] = make_cconst<R>(f.x, f.y);
+    }
+  }
+  auto kin1_at = [&](int i) -> cconst<R> {
+    const int f = freq_index(i, n);
+    const int af = f < 0 ? -f : f;
+    if constexpr (kStageKin1) {
+      return kin1_lds[af];
+    } else {
+      const cpx<double> w = gload(s.kin1 + af);
+      return make_cconst<R>(w.x, w.y);
+    }
+  };
   auto kin_at = [&](int i) -> cpx<double> {
     if constexpr (kStageKin) return kin_lds[i];
     else return gload(s.kin_x + i);
@@ -644,6 +679,100 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         with_vol(std::false_type{}, std::false_type{});
       }
       if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+    } else if constexpr (kKin1) {
+      // The separable plan.  Z kinds open with the local half-step(s) in real
+      // space, XK_LAST closes with one after its inverse transform; between,
+      // FFT along the pass's axis, that axis's 1D kinetic factor, inverse FFT.
+      constexpr bool kLocalZ = KIND == K_ZK_FIRST || KIND == K_ZK_MID;
+      constexpr bool kLocalX = KIND == K_XK_LAST;
+      constexpr bool kMerged = KIND == K_ZK_MID;
+      auto local = [&](auto imag_tag, auto vim_tag, auto vol_tag) {
+        constexpr bool IMAG = decltype(imag_tag)::value;
+        constexpr bool VIM = decltype(vim_tag)::value;
+        constexpr bool VOL = decltype(vol_tag)::value;
+        // the pencil's share of the potential (Z: its (x, y) entry; X: its z
+        // entry) and, per point, the line along the pass's axis (Z: pz(z);
+        // X: pxy(x, y) at the block's y); summed first, then scaled by dt/2
+        R pen;
+        if constexpr (kLocalZ) {
+          pen = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)pi * n + pj));
+          if constexpr (kMerged) pen += gload(s.pxy + (a.slot_b * s.pxy_ss + (long long)pi * n + pj));
+        } else {
+          pen = gload(s.pz + (a.slot_a * s.pz_ss + pk));
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) {
+            const int idx = TF::pos(t, m);
+            R line;
+            if constexpr (kLocalZ) {
+              line = gload(s.pz + (a.slot_a * s.pz_ss + idx));
+              if constexpr (kMerged) line += s.pz_ss == 0 ? line : gload(s.pz + (a.slot_b * s.pz_ss + idx));
+            } else {
+              line = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)idx * n + pj));
+            }
+            R ph = pen + line;
+            const long long vidx = g.base + (long long)idx * stride;
+            if constexpr (VOL) {
+              R pv = R(0);
+              if (s.vre) pv = gload(s.vre + vidx);
+              if constexpr (kMerged) pv += pv;
+              if (s.vdyn) {
+                pv += gload(s.vdyn + (a.slot_a * s.vdyn_ss + vidx));
+                if constexpr (kMerged) pv += gload(s.vdyn + (a.slot_b * s.vdyn_ss + vidx));
+              }
+              ph += pv;
+            }
+            ph = ph * s.hdt;
+            R vi = R(0);
+            if constexpr (VIM) vi = gload(s.vim + vidx);
+            cpx<R> w;
+            if constexpr (kMerged) w = two_half_steps<VIM>(v[m], ph, vi, s);
+            else w = half_step<IMAG, VIM>(v[m], ph, vi, s);
+            if constexpr (kLocalX) {
+              if (norm) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+            }
+            v[m] = w;
+          }
+        }
+      };
+      const bool vol = s.vre != nullptr || s.vdyn != nullptr;
+      auto with_vol = [&](auto imag_tag, auto vim_tag) {
+        if (vol) local(imag_tag, vim_tag, std::true_type{});
+        else local(imag_tag, vim_tag, std::false_type{});
+      };
+      // merged half-steps are real-time only; an absorber exists only in real time
+      auto half_steps = [&]() {
+        if (!kMerged && s.imag) {
+          if constexpr (!kMerged) with_vol(std::true_type{}, std::false_type{});
+        } else if (has_vim) {
+          with_vol(std::false_type{}, std::true_type{});
+        } else {
+          with_vol(std::false_type{}, std::false_type{});
+        }
+      };
+      if constexpr (kLocalZ) {
+        if constexpr (KIND == K_ZK_FIRST) {
+          if (prescale) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
+          }
+        }
+        half_steps();
+      }
+      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      // the axis's factor, applied with one rounding per component
+      if (s.imag) {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          if (TF::valid(t, m, n)) v[m] = scale_re(v[m], kin1_at(TF::pos(t, m)));
+      } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          if (TF::valid(t, m, n)) v[m] = cmul_c(v[m], kin1_at(TF::pos(t, m)));
+      }
+      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      if constexpr (kLocalX) half_steps();
     } else if constexpr (KIND == K_Z_DIAG) {
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
       if (prescale) {
@@ -726,13 +855,22 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // ---- per-block reductions
   const unsigned nblk = gridDim.x;
   const unsigned bid = blockIdx.x;
-  if constexpr (KIND == K_Z_LAST) {
+  if constexpr (KIND == K_Z_LAST || KIND == K_XK_LAST) {
     if (a.flags & F_NORM) {
       // FP64 block partial only; k_norm_finish (launched right after on the
       // same stream) sums the partials.  An agent-scope fence + ticket here
       // would write back the XCD's L2 once per block, which costs more than
-      // the tile itself on small grids.
-      __shared__ double red_scratch[16];
+      // the tile itself on small grids.  XK_LAST sums through the exchange
+      // buffer, free after its last transform (block_sum's first barrier
+      // orders the exchange reads before the sums' writes), which keeps the
+      // pass within its LDS budget.
+      double* red_scratch;
+      if constexpr (KIND == K_XK_LAST) {
+        red_scratch = reinterpret_cast<double*>(lds);
+      } else {
+        __shared__ double red_lds[16];
+        red_scratch = red_lds;
+      }
       double part[1] = {acc[0]};
       block_sum<1>(part, red_scratch);
       double* red = a.red ? a.red : s.red;

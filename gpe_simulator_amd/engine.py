@@ -33,8 +33,10 @@ def make_solve(n, x_array, k_array, kind, dt, beta, static=None, dynamic=None, k
     None); kcut: cutoff wavenumber (real time only) or None."""
     kind_code = _lib.IMAG if kind == "imaginary" else _lib.REAL
     ctx = ctx or _lib.default_context()
-    # pass variants: tuned once per context and shape, then the context default
-    tune.ensure_tuned(n, precision_code(precision), kind_code, ctx)
+    # pass variants: tuned once per context, shape and step plan, then the
+    # context default
+    plan_code = _lib.default_plan(precision_code(precision), kind_code, masked=kcut is not None)
+    tune.ensure_tuned(n, precision_code(precision), kind_code, ctx, plan=plan_code)
     solve = _lib.Solve(n, precision_code(precision), kind_code, dt, beta, x_array, k_array, kcut=kcut, ctx=ctx)
     plan = plan_potential(static, dynamic if kind_code == _lib.REAL else None)
     for target, program in plan.static.items():

@@ -1,10 +1,12 @@
 """Autotuner for the pass tile variants.
 
-Every pass kind of a step (z_first, z_mid, z_last, y_fwd, y_inv, x_kin) reads
-and writes the field in the same canonical layout, so each kind can run any of
-the compiled tile variants of its (side, precision): threads per pencil,
-pencils per block and the waves per SIMD its register budget allows
-(gpe_shapes.h).  The tuner works on scratch solves of the real shape (alone,
+Every pass kind of a step (separable plan: zk_first, zk_mid, yk, xk, xk_last;
+four-sweep plan: z_first, z_mid, z_last, y_fwd, y_inv, x_kin) reads and writes
+the field in the same canonical layout, so each kind can run any of the
+compiled tile variants of its (side, precision): threads per pencil, pencils
+per block and the waves per SIMD its register budget allows (gpe_shapes.h).
+Each step plan is tuned on its own, when a solve of that plan is first made.
+The tuner works on scratch solves of the real shape (alone,
 or as a batch of `batch` members, since small grids behave differently when
 many solves share a launch) in two stages:
 
@@ -35,10 +37,15 @@ from dataclasses import dataclass, field
 from . import _lib
 from .grids import get_grids
 
-STEP_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin")
+STEP_KINDS = {_lib.PLAN_SEPARABLE: ("zk_first", "zk_mid", "yk", "xk", "xk_last"),
+              _lib.PLAN_FOUR_SWEEP: ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin")}
 # pass kinds of a step in the order the descent visits them (most time first)
-STEP_SEQUENCE = {_lib.REAL: ("z_mid", "x_kin", "y_fwd", "y_inv", "z_first", "z_last"),
-                 _lib.IMAG: ("z_first", "z_last", "x_kin", "y_fwd", "y_inv")}
+STEP_SEQUENCE = {
+    _lib.PLAN_SEPARABLE: {_lib.REAL: ("zk_mid", "xk", "yk", "zk_first", "xk_last"),
+                          _lib.IMAG: ("xk_last", "zk_first", "yk")},
+    _lib.PLAN_FOUR_SWEEP: {_lib.REAL: ("z_mid", "x_kin", "y_fwd", "y_inv", "z_first", "z_last"),
+                           _lib.IMAG: ("z_first", "z_last", "x_kin", "y_fwd", "y_inv")},
+}
 GAIN = 0.005        # a descent move must shorten the step by more than this fraction
 _lock = threading.Lock()
 
@@ -49,6 +56,7 @@ class Tuning:
     precision: int
     kind: int
     batch: int
+    plan: int
     variants: list                       # [(tp, pencils per block, waves)]
     choice: list                         # variant per pass kind (PASS_KINDS order)
     timings: dict = field(default_factory=dict)   # variant -> {kind: avg us}
@@ -57,13 +65,15 @@ class Tuning:
     event_step_us: float = float("nan")  # whole-step time of `event_choice`
 
     def report(self):
+        kinds = STEP_KINDS[self.plan]
         lines = [f"n={self.n} precision={'c64' if self.precision == _lib.C64 else 'c128'} "
-                 f"kind={'imaginary' if self.kind == _lib.IMAG else 'real'} batch={self.batch}"]
+                 f"kind={'imaginary' if self.kind == _lib.IMAG else 'real'} batch={self.batch} "
+                 f"plan={'separable' if self.plan == _lib.PLAN_SEPARABLE else 'four-sweep'}"]
         for v, geo in enumerate(self.variants):
             t = self.timings.get(v, {})
             lines.append(f"  v{v} tp={geo[0]} b={geo[1]} waves={geo[2]}: " +
-                         ", ".join(f"{k} {t[k]:.2f}" for k in STEP_KINDS if k in t))
-        lines.append("  choice: " + ", ".join(f"{k}=v{self.choice[_lib.PASS_KINDS.index(k)]}" for k in STEP_KINDS)
+                         ", ".join(f"{k} {t[k]:.2f}" for k in kinds if k in t))
+        lines.append("  choice: " + ", ".join(f"{k}=v{self.choice[_lib.PASS_KINDS.index(k)]}" for k in kinds)
                      + f" ({self.step_us:.2f} us/step; per-pass choice {self.event_step_us:.2f})")
         return "\n".join(lines)
 
@@ -72,10 +82,11 @@ def enabled():
     return os.environ.get("GPE_HIP_AUTOTUNE", "1") != "0"
 
 
-def _scratch(n, precision, kind, ctx):
+def _scratch(n, precision, kind, ctx, plan):
     length = 20.0
     x, k = get_grids(n, length)
     s = _lib.Solve(n, precision, kind, 5e-4, 100.0, x, k, ctx=ctx)
+    s.set_plan(plan)
     from .expressions import plan_potential
     for target, program in plan_potential("x**2 + y**2 + z**2", None).static.items():
         s.potential_static(target, program)
@@ -83,12 +94,12 @@ def _scratch(n, precision, kind, ctx):
     return s
 
 
-def measure(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
-    """Average µs per launch of every step pass kind, for every variant:
-    {variant: {kind: us}} (the minimum over `rounds` timed runs)."""
+def measure(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None, plan=_lib.PLAN_SEPARABLE):
+    """Average µs per launch of every step pass kind of `plan`, for every
+    variant: {variant: {kind: us}} (the minimum over `rounds` timed runs)."""
     ctx = ctx or _lib.default_context()
     variants = _lib.pass_variants(n, precision)
-    solves = [_scratch(n, precision, kind, ctx) for _ in range(batch)]
+    solves = [_scratch(n, precision, kind, ctx, plan) for _ in range(batch)]
     group = _lib.Batch(solves, ctx=ctx) if batch > 1 else None
     out = {}
     try:
@@ -104,7 +115,7 @@ def measure(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
                     solves[0].step(2)
                     timing = solves[0].step_timed(steps)
                 for name, (ms, count) in timing.items():
-                    if count and name in STEP_KINDS:
+                    if count and name in STEP_KINDS[plan]:
                         us = ms / count * 1e3
                         best[name] = min(best.get(name, us), us)
             out[v] = best
@@ -121,8 +132,8 @@ class StepClock:
     device time between two marks on a solo solve's stream, or host time
     around synchronous graph replays of a batch."""
 
-    def __init__(self, n, precision, kind, batch, ctx, steps=None, rounds=3):
-        self.solves = [_scratch(n, precision, kind, ctx) for _ in range(batch)]
+    def __init__(self, n, precision, kind, batch, ctx, steps=None, rounds=3, plan=_lib.PLAN_SEPARABLE):
+        self.solves = [_scratch(n, precision, kind, ctx, plan) for _ in range(batch)]
         self.group = _lib.Batch(self.solves, ctx=ctx) if batch > 1 else None
         # about 2e8 grid-point-steps (a few ms) per timed round
         self.steps = steps or max(8, min(100, int(2e8 / (n ** 3 * batch))))
@@ -155,16 +166,25 @@ class StepClock:
             s.close()
 
 
-def refine(clock, start, kind, nvariants):
-    """Fastest whole step from `start` and the uniform choices, improved by
-    coordinate descent over the step's pass kinds (two sweeps, moves that
-    gain more than GAIN).  Returns (choice, us per step)."""
-    candidates = [list(start)] + [[v] * len(_lib.PASS_KINDS) for v in range(nvariants)]
+def refine(clock, start, kind, nvariants, plan=_lib.PLAN_SEPARABLE):
+    """Fastest whole step from `start` and the uniform choices (over the plan's
+    pass kinds; other kinds keep `start`), improved by coordinate descent over
+    the step's pass kinds (two sweeps, moves that gain more than GAIN).
+    Returns (choice, us per step)."""
+    idxs = [_lib.PASS_KINDS.index(k) for k in STEP_KINDS[plan]]
+
+    def uniform(v):
+        c = list(start)
+        for i in idxs:
+            c[i] = v
+        return c
+
+    candidates = [list(start)] + [uniform(v) for v in range(nvariants)]
     timed = [(clock(c), c) for c in candidates]
     best_us, best = min(timed, key=lambda item: item[0])
     for _ in range(2):
         moved = False
-        for name in STEP_SEQUENCE[kind]:
+        for name in STEP_SEQUENCE[plan][kind]:
             idx = _lib.PASS_KINDS.index(name)
             for v in range(nvariants):
                 if v == best[idx]:
@@ -179,45 +199,52 @@ def refine(clock, start, kind, nvariants):
     return best, best_us
 
 
-def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None):
-    """Time every variant per pass, then decide on whole-step time (a Tuning)."""
+def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None, plan=None, start=None):
+    """Time every variant per pass of `plan` (default: the plan new solves of
+    this precision and kind step with), then decide on whole-step time (a
+    Tuning).  Kinds outside the plan keep `start` (default: variant 0)."""
     precision = int(precision)
     kind = int(kind)
+    plan = _lib.default_plan(precision, kind) if plan is None else int(plan)
     ctx = ctx or _lib.default_context()
-    variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx)
-    event_choice = [0] * len(_lib.PASS_KINDS)
-    for name in STEP_KINDS:
+    variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx, plan=plan)
+    event_choice = list(start) if start is not None else [0] * len(_lib.PASS_KINDS)
+    for name in STEP_KINDS[plan]:
         measured = [(timings[v][name], v) for v in timings if name in timings[v]]
         if measured:
             event_choice[_lib.PASS_KINDS.index(name)] = min(measured)[1]
-    clock = StepClock(n, precision, kind, batch, ctx)
+    clock = StepClock(n, precision, kind, batch, ctx, plan=plan)
     try:
         event_us = clock(event_choice)
-        choice, step_us = refine(clock, event_choice, kind, len(variants))
+        choice, step_us = refine(clock, event_choice, kind, len(variants), plan=plan)
     finally:
         clock.close()
-    return Tuning(n=n, precision=precision, kind=kind, batch=batch, variants=variants, choice=choice,
+    return Tuning(n=n, precision=precision, kind=kind, batch=batch, plan=plan, variants=variants, choice=choice,
                   timings=timings, event_choice=event_choice, step_us=step_us, event_step_us=event_us)
 
 
-def ensure_tuned(n, precision, kind, ctx=None, batch=1):
-    """Tune (side, precision, kind) once per context and, for solo tuning,
-    make the choice the context's default for new solves unless the caller
+def ensure_tuned(n, precision, kind, ctx=None, batch=1, plan=None):
+    """Tune (side, precision, kind) for `plan` once per context and, for solo
+    tuning, make the choice the context's default for new solves (the plan's
+    kinds only; a tuning of the other plan keeps its own) unless the caller
     has set that default.  Returns the Tuning, or None when tuning is
     disabled, the side has a single variant or the default was set."""
     if not enabled():
         return None
     ctx = ctx or _lib.default_context()
     n, precision, kind = int(n), int(precision), int(kind)
-    key = (n, precision, kind, int(batch))
+    plan = _lib.default_plan(precision, kind) if plan is None else int(plan)
+    key = (n, precision, kind, int(batch)) if plan == _lib.default_plan(precision, kind) else \
+        (n, precision, kind, int(batch), plan)
     with _lock:
         if key in ctx.tunings:
             return ctx.tunings[key]
         if len(_lib.pass_variants(n, precision)) < 2 or (batch == 1 and key[:3] in ctx.variant_defaults):
             ctx.tunings[key] = None
             return None
-        result = autotune(n, precision, kind, batch=batch, ctx=ctx)
+        result = autotune(n, precision, kind, batch=batch, ctx=ctx, plan=plan,
+                          start=ctx.default_variants(n, precision, kind))
         if batch == 1:
-            ctx.set_variants(n, precision, kind, result.choice)
+            ctx.set_variants(n, precision, kind, result.choice, tuned=True)
         ctx.tunings[key] = result
         return result

@@ -61,11 +61,11 @@ int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime
 
 /* Pass tile variants.  A step is a sequence of passes (kinds 0..GPE_PASS_KINDS-1:
  * z_first, z_mid, z_last, y_fwd, y_inv, x_kin, z_diag, x_diag, z_fwd, y_lz,
- * x_lz); each pass kind of a solve runs one of the compiled tile variants of
+ * x_lz, zk_first, zk_mid, yk, xk, xk_last); each pass kind of a solve runs one of the compiled tile variants of
  * its (side, precision), all producing the same transform.  An autotuner
  * times the variants and sets the fastest per kind.  No reference counterpart:
  * the reference's FFT plans are cuFFT's (gpe/make_propagator.py:50,52). */
-enum { GPE_PASS_KINDS = 11 };
+enum { GPE_PASS_KINDS = 16 };
 /* Bins of the equal-width spectrum view (gpe_spectrum_view). */
 enum { GPE_VIEW_BINS = 128 };
 /* Number of compiled variants for (side, precision); 1 for sides without a
@@ -105,6 +105,25 @@ int gpe_solve_destroy(gpe_solve* solve);
 /* Device bytes a solve of this description holds (psi + tables), and the
  * extra bytes the diagnostics workspace adds when first used. */
 int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size_t* diag_bytes);
+/* Step plan of a solve.  Both give the reference's Strang step
+ * (gpe/make_propagator.py:38-152) to rounding.  GPE_PLAN_SEPARABLE, the
+ * default: the kinetic factor exp(-i k^2 dt) is the product of one factor per
+ * axis, and each commutes with the transforms along the other axes, so a step
+ * is three passes (z, y, x), each FFT -> 1D factor -> inverse FFT, with the
+ * local half-steps fused into the z pass (and the last one into the x pass):
+ * 3 sweeps of psi per step.  GPE_PLAN_FOUR_SWEEP: the 3D FFT, the kinetic
+ * factor and the inverse 3D FFT in four passes (z, y, x, y; 5 in imaginary
+ * time).  A momentum cutoff (kcut_enable) is not separable, so such solves
+ * use, and may only use, the four-sweep plan.  Defaults: separable for
+ * imaginary time and for complex128; four-sweep for complex64 real time,
+ * where the three separately rounded small rotations per step make the norm
+ * drift (rounding leans outward for tiny angles).  Batch members share a
+ * plan. */
+#define GPE_PLAN_SEPARABLE 0
+#define GPE_PLAN_FOUR_SWEEP 1
+int gpe_solve_set_plan(gpe_solve* solve, int32_t plan);
+int gpe_solve_get_plan(gpe_solve* solve, int32_t* plan);
+
 /* Pass variant per kind of one solve (GPE_PASS_KINDS entries each).  Batch
  * members must use the same variants. */
 int gpe_solve_set_variants(gpe_solve* solve, const int32_t* variants, int32_t nkinds);

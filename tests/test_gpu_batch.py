@@ -187,7 +187,7 @@ def test_server_warm_up_tunes_every_shape(require_gpu):
     for n, prec, kind in shapes:
         key = (n, _lib.C64 if prec == "c64" else _lib.C128, _lib.IMAG if kind == "imaginary" else _lib.REAL, 1)
         assert ctx.tunings[key] is not None
-        assert ctx.variant_defaults[key[:3]] == ctx.tunings[key].choice
+        assert ctx.default_variants(*key[:3]) == ctx.tunings[key].choice
     again = server.warm(shapes)
     assert again < seconds
     ctx.close()

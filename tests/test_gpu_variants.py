@@ -55,9 +55,11 @@ def test_every_variant_matches_the_oracle(n, precision, require_gpu):
         s = make_solve(n, real.x, real.k, "real", dt, beta, static=real.static_source, dynamic=BEAM,
                        precision=precision)
         s.set_variants([v] * len(_lib.PASS_KINDS))
-        s.upload(psi0)
-        s.step(steps, midpoint_times(0, steps, dt))
-        assert rel_l2(s.to_numpy(), exp_real) < tol, f"variant {v}"
+        for plan in (_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE):     # both step plans, default last
+            s.set_plan(plan)
+            s.upload(psi0)
+            s.step(steps, midpoint_times(0, steps, dt))
+            assert rel_l2(s.to_numpy(), exp_real) < tol, f"variant {v} plan {plan}"
         # diagnostic passes on the same variant, on the initial state
         s.upload(psi0)
         d = s.diagnostics(t)
@@ -69,9 +71,11 @@ def test_every_variant_matches_the_oracle(n, precision, require_gpu):
         s.close()
         s = make_solve(n, imag.x, imag.k, "imaginary", dt, beta, static=imag.static_source, precision=precision)
         s.set_variants([v] * len(_lib.PASS_KINDS))
-        s.upload(psi0)
-        s.step(steps)
-        assert rel_l2(s.to_numpy(), exp_imag) < tol, f"variant {v}"
+        for plan in (_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE):
+            s.set_plan(plan)
+            s.upload(psi0)
+            s.step(steps)
+            assert rel_l2(s.to_numpy(), exp_imag) < tol, f"variant {v} plan {plan}"
         s.close()
 
 
