@@ -222,6 +222,110 @@ def stirring(count, n=128, steps=500):
             "Lz_range": [min(r["Lz"] for r in batched), max(r["Lz"] for r in batched)]}
 
 
+def probe_small_grids():
+    """Per shape (64, 96, 128; both precisions and kinds): device time per
+    step of a solo solve between two marks, the sum of its kernels' own
+    durations per step, and host time per step of the same solve as a
+    one-member batch replayed from a hipGraph and launched eagerly."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.grids import get_grids
+    rows = []
+    for n in (64, 96, 128):
+        for prec in ("c64", "c128"):
+            for kind in ("real", "imaginary"):
+                x, k = get_grids(n, 20.0)
+                s = make_solve(n, x, k, kind, 5e-4, 300.0, static="x**2 + y**2 + z**2", precision=prec)
+                s.gaussian(1.0)
+                s.step(20)
+                steps = 400
+                s.mark(0)
+                s.step(steps)
+                s.mark(1)
+                solo = s.marked_ms() * 1e3 / steps
+                kernels = sum(v[0] for v in s.step_kernel_times(steps).values()) / steps
+                b = _lib.Batch([s])
+                b.step(steps, graph=True)
+                t0 = time.perf_counter()
+                b.step(steps, graph=True)
+                graph = (time.perf_counter() - t0) * 1e6 / steps
+                t0 = time.perf_counter()
+                b.step(steps, graph=False)
+                eager = (time.perf_counter() - t0) * 1e6 / steps
+                b.close()
+                rows.append({"n": n, "precision": prec, "kind": kind, "plan": s.plan, "solo_us": solo,
+                             "kernels_us": kernels, "batch1_graph_us": graph, "batch1_eager_us": eager})
+                s.close()
+    return rows
+
+
+def probe_mixed_sides(steps=200):
+    """Solves of different sides advance as one batch per side, each on its
+    own stream.  Grid-point-steps/s of three concurrent batches (8 x 64^3,
+    8 x 96^3, 2 x 128^3; imaginary time, complex64) against each batch alone
+    and against one equal-side batch of about the same points (16 x 96^3)."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve
+    from gpe_simulator_amd.grids import get_grids
+
+    def batch(n, count):
+        x, k = get_grids(n, 20.0)
+        solves = [make_solve(n, x, k, "imaginary", 5e-4, 300.0, static="x**2 + y**2 + z**2", precision="c64")
+                  for _ in range(count)]
+        for s in solves:
+            s.gaussian(1.0)
+        return _lib.Batch(solves), solves, n ** 3 * count
+
+    groups = [batch(64, 8), batch(96, 8), batch(128, 2)]
+    single = batch(96, 16)
+    out = {"steps": steps}
+    for b, _, _ in groups + [single]:
+        b.step(steps, graph=True)                      # capture and warm
+    for b, _, pts in groups + [single]:
+        t0 = time.perf_counter()
+        b.step(steps, graph=True)
+        out.setdefault("alone", []).append({"points": pts, "gpts_per_s": pts * steps / (time.perf_counter() - t0)})
+    t0 = time.perf_counter()
+    for b, _, _ in groups:
+        b.launch(steps, graph=True)
+    for b, _, _ in groups:
+        b.wait()
+    el = time.perf_counter() - t0
+    total = sum(p for _, _, p in groups)
+    out["concurrent"] = {"points": total, "gpts_per_s": total * steps / el,
+                         "sum_of_alone_s": sum(p * steps / a["gpts_per_s"] for (_, _, p), a in
+                                               zip(groups, out["alone"][:3])), "seconds": el}
+    out["one_equal_side_batch"] = out["alone"][3]
+    for b, solves, _ in groups + [single]:
+        b.close()
+        for s in solves:
+            s.close()
+    return out
+
+
+def probe_large_grids(steps=20):
+    """ms per step of the stirring workload's complex64 and complex128 runs at
+    512^3 (L = 80, the headline's spacing) against 8x the 256^3 step."""
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    from gpe_simulator_amd.grids import get_grids
+    out = {}
+    for n, length in ((256, LENGTH), (512, 2 * LENGTH)):
+        x, k = get_grids(n, length)
+        for prec in ("c64", "c128"):
+            s = make_solve(n, x, k, "real", DT, BETA, static=STATIC, dynamic=BEAM, precision=prec)
+            s.gaussian(2.0)
+            tm = midpoint_times(0, steps + 2, DT)
+            s.step(2, tm[:2])
+            s.mark(0)
+            s.step(steps, tm[2:])
+            s.mark(1)
+            out[f"{prec}_{n}_ms"] = s.marked_ms() / steps
+            s.close()
+    for prec in ("c64", "c128"):
+        out[f"{prec}_512_over_8x256"] = out[f"{prec}_512_ms"] / (8 * out[f"{prec}_256_ms"])
+    return out
+
+
 def cpu_baseline(psi0, samples, workers=None):
     """The FP64 oracle on a bounded sample of the workload: numpy.fft on one
     thread, or scipy.fft with `workers` threads.  Steps 0..samples from psi0
@@ -304,7 +408,14 @@ def main(argv=None):
     parser.add_argument("--serve-requests", type=int, default=24, help="0 skips the serving stream")
     parser.add_argument("--sweep-count", type=int, default=64, help="0 skips the batched sweep")
     parser.add_argument("--stir-count", type=int, default=16, help="0 skips the stirring-speed sweep")
+    parser.add_argument("--probe", choices=("small", "mixed", "large"), default=None,
+                        help="print one JSON line of a side measurement instead of the bench line: small-grid "
+                             "step cost, concurrent batches of different sides, or 512^3 against 256^3")
     args = parser.parse_args(argv)
+    if args.probe:
+        fn = {"small": probe_small_grids, "mixed": probe_mixed_sides, "large": probe_large_grids}[args.probe]
+        print(json.dumps({"probe": args.probe, "result": fn()}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
               "the engine runs a solve on one GPU and has no multi-GPU path", file=sys.stderr)

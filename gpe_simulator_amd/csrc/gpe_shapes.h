@@ -26,11 +26,15 @@
   X(0, 128, 0, 8, 32, 3, 1) X(0, 128, 1, 8, 16, 3, 1) X(0, 128, 2, 8, 16, 4, 1)                           \
   X(0, 192, 0, 8, 16, 3, 1) X(0, 192, 1, 8, 32, 3, 1) X(0, 192, 2, 8, 16, 4, 1) X(0, 192, 3, 8, 16, 3, 2) \
   X(0, 256, 0, 16, 16, 3, 1) X(0, 256, 1, 16, 16, 4, 1) X(0, 256, 2, 16, 32, 3, 1)                        \
+  X(0, 512, 0, 16, 8, 2, 1) X(0, 512, 1, 16, 16, 2, 1)                                                     \
+  X(0, 1024, 0, 32, 4, 1, 1) X(0, 1024, 1, 32, 8, 1, 1)                                                    \
   X(1, 64, 0, 8, 16, 2, 1) X(1, 64, 1, 8, 32, 2, 1)                                                       \
   X(1, 96, 0, 4, 16, 2, 1) X(1, 96, 1, 4, 32, 2, 1) X(1, 96, 2, 4, 8, 2, 1) X(1, 96, 3, 4, 32, 2, 2)      \
   X(1, 128, 0, 8, 16, 2, 1) X(1, 128, 1, 8, 32, 2, 1) X(1, 128, 2, 8, 8, 2, 1)                            \
   X(1, 192, 0, 8, 8, 2, 1) X(1, 192, 1, 8, 16, 2, 1) X(1, 192, 2, 8, 16, 2, 2)                            \
-  X(1, 256, 0, 16, 8, 2, 1) X(1, 256, 1, 16, 16, 2, 1) X(1, 256, 2, 16, 4, 2, 1)
+  X(1, 256, 0, 16, 8, 2, 1) X(1, 256, 1, 16, 16, 2, 1) X(1, 256, 2, 16, 4, 2, 1)                          \
+  X(1, 512, 0, 16, 4, 1, 1) X(1, 512, 1, 16, 8, 1, 1)                                                      \
+  X(1, 1024, 0, 32, 2, 1, 1) X(1, 1024, 1, 32, 4, 1, 1)
 
 namespace gpe {
 

@@ -249,3 +249,66 @@ def test_stirring_50_steps_c128_at_256(require_gpu):
             assert e_dev["mu"] == pytest.approx(e_ref["mu"], rel=1e-6)
             assert rel_l2(solve.to_numpy(), expected) < 1e-4
         solve.close()
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_fast_512_matches_the_oracle(precision, require_gpu):
+    """The 512 transform (16 threads per pencil, 32 values each) under every
+    compiled variant and both step plans: one real-time step with the
+    moving beam and one imaginary-time step against the FP64 oracle
+    (complex128: relative L2 1e-11; complex64: 2e-5)."""
+    from gpe_simulator_amd import _lib
+    from gpe_simulator_amd.engine import make_solve, midpoint_times
+    n, length, dt, beta = 512, 48.0, 5e-4, 400.0
+    real = orc.Problem(n, length, "real", dt, beta, static="x**2 + y**2 + 2*z**2", dynamic=BEAM)
+    psi0 = orc.synthetic_state(n, length, seed=9, beta=beta)
+    expected = threaded_run(real, psi0, 1)
+    imag = orc.Problem(n, length, "imaginary", dt, beta, static="x**2 + y**2 + 2*z**2")
+    expected_imag = imag.step(psi0.copy())
+    tol = 1e-11 if precision == "c128" else 2e-5
+    nvar = len(_lib.pass_variants(n, _lib.C64 if precision == "c64" else _lib.C128))
+    assert nvar >= 2
+    for v in range(nvar):
+        print(f"512 {precision} variant {v}", flush=True)
+        s = make_solve(n, real.x, real.k, "real", dt, beta, static=real.static_source, dynamic=BEAM,
+                       precision=precision)
+        s.set_variants([v] * len(_lib.PASS_KINDS))
+        for plan in (_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE):
+            s.set_plan(plan)
+            s.upload(psi0)
+            s.step(1, midpoint_times(0, 1, dt))
+            assert rel_l2(s.to_numpy(), expected) < tol, (v, plan)
+        s.close()
+        s = make_solve(n, imag.x, imag.k, "imaginary", dt, beta, static=imag.static_source, precision=precision)
+        s.set_variants([v] * len(_lib.PASS_KINDS))
+        s.upload(psi0)
+        s.step(1)
+        assert rel_l2(s.to_numpy(), expected_imag) < tol, v
+        s.close()
+
+
+@pytest.mark.parametrize("precision", ["c64", "c128"])
+def test_free_gaussian_at_1024(precision, require_gpu):
+    """The 1024 transform on the reference's largest benchmarked grid: a
+    Gaussian (sigma = 1.5) on [-32, 32)^3 with no potential and no
+    interaction spreads as the closed form
+    psi = (pi s0)^(-3/4) (s0/s)^(3/2) exp(-r^2 / (2 s)), s = s0 + 2 i t
+    (i dpsi/dt = -lap psi; the spectral step is exact for it), checked on the
+    x = 0 plane after 40 steps of dt = 1e-3, with the norm kept."""
+    from gpe_simulator_amd.engine import make_solve
+    n, length, dt, steps, sigma = 1024, 64.0, 1e-3, 40, 1.5
+    x, k = orc.get_grids(n, length)
+    s = make_solve(n, x, k, "real", dt, 0.0, precision=precision)
+    s.gaussian(sigma)
+    n0 = s.norm()
+    s.step(steps)
+    plane = s.download((n // 2) * n * n, n * n).reshape(n, n)   # psi[x = 0, :, :]
+    t = steps * dt
+    s0 = sigma ** 2
+    st = s0 + 2j * t
+    Y, Z = x[:, None], x[None, :]
+    exact = (np.pi * s0) ** -0.75 * (s0 / st) ** 1.5 * np.exp(-(Y * Y + Z * Z) / (2 * st))
+    tol = 1e-10 if precision == "c128" else 1e-5
+    assert np.max(np.abs(plane - exact)) < tol * np.abs(exact).max()
+    assert s.norm() == pytest.approx(n0, rel=1e-12 if precision == "c128" else 1e-5)
+    s.close()
