@@ -487,6 +487,27 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       pz_lds[e] = pz;
     }
   }
+  // The separable plan's line of the potential along the pass's axis (Z
+  // kinds: pz(z), both slots summed for ZK_MID; XK_LAST: the block's pxy(x, y)
+  // column, every pencil of an X block sharing its y), staged in LDS for
+  // complex128, where the LDS budget allows it; complex64 reads it from the
+  // cache, which keeps its Z and X passes at four blocks per CU.
+  constexpr bool kStageLine = TF::kFast && sizeof(R) == 8 &&
+                              (KIND == K_ZK_FIRST || KIND == K_ZK_MID || KIND == K_XK_LAST);
+  __shared__ R line_lds[kStageLine ? TF::N : 1];
+  if constexpr (kStageLine) {
+    if constexpr (KIND == K_XK_LAST) {
+      const int pj_block = int(blockIdx.x) / (TF::N / TF::B);   // the X tile's y (Tile<TF, 0>)
+      for (int e = threadIdx.x; e < TF::N; e += TF::THREADS)
+        line_lds[e] = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)e * TF::N + pj_block));
+    } else {
+      for (int e = threadIdx.x; e < TF::N; e += TF::THREADS) {
+        R pz = gload(s.pz + (a.slot_a * s.pz_ss + e));
+        if constexpr (KIND == K_ZK_MID) pz += gload(s.pz + (a.slot_b * s.pz_ss + e));
+        line_lds[e] = pz;
+      }
+    }
+  }
   // the separable plan's 1D factor, by |frequency| (n/2 + 1 entries)
   constexpr bool kKin1 =
       KIND == K_ZK_FIRST || KIND == K_ZK_MID || KIND == K_YK || KIND == K_XK || KIND == K_XK_LAST;
@@ -705,7 +726,9 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           if (TF::valid(t, m, n)) {
             const int idx = TF::pos(t, m);
             R line;
-            if constexpr (kLocalZ) {
+            if constexpr (kStageLine) {
+              line = line_lds[idx];
+            } else if constexpr (kLocalZ) {
               line = gload(s.pz + (a.slot_a * s.pz_ss + idx));
               if constexpr (kMerged) line += s.pz_ss == 0 ? line : gload(s.pz + (a.slot_b * s.pz_ss + idx));
             } else {
