@@ -46,13 +46,12 @@ STATIC = "x**2 + y**2 + z**2"
 BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
 NK_INTERVAL = 500
 HBM_PEAK_GBS = 8000.0
-# steady-state psi sweeps per real-time step: the complex64 run steps with the
-# four-sweep plan (3D FFT, kinetic factor, inverse 3D FFT in passes z, y, x, y
-# with the half-steps fused into z); the separable plan (one pass per axis,
-# each FFT -> 1D kinetic factor -> inverse FFT) needs 3 but rounds three
-# rotations per point and step, so complex64 real time keeps four
-PASSES_PER_STEP = 4
-PASSES_SEPARABLE = 3
+# steady-state psi sweeps per real-time step: the separable plan (the
+# default) runs one pass per axis (z, y, x), each FFT -> 1D kinetic factor ->
+# inverse FFT, with the local half-steps fused into the z pass; the four-sweep
+# plan (3D FFT, kinetic factor, inverse 3D FFT in passes z, y, x, y) needs 4
+PASSES_PER_STEP = 3
+PASSES_FOUR_SWEEP = 4
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_stirring256_c64.txt"
 STATS_FILE = ROOT / "profiles" / "kernel_stats_stirring256.md"
 
@@ -450,26 +449,29 @@ def main(argv=None):
     ms_copy = _lib.default_context().bench_copy(copy_bytes, reps=20)
     copy_gbs = 2 * copy_bytes / (ms_copy * 1e-3) / 1e9
 
-    result_c128, acc = None, None
+    result_c128, acc, four64 = None, None, None
     c128_steps = args.c128_steps or args.steps
     if not args.skip_c128:
         from gpe_simulator_amd import _lib
-        log(f"c128 run: {c128_steps} steps, the complex64 run's four-sweep plan")
-        solve128, el128, diag128 = run_engine("c128", psi0, x, k, c128_steps, args.warmup,
-                                              plan=_lib.PLAN_FOUR_SWEEP)
+        log(f"c128 run: {c128_steps} steps, the same (separable) plan as the complex64 run")
+        solve128, el128, diag128 = run_engine("c128", psi0, x, k, c128_steps, args.warmup)
         v128 = N ** 3 * c128_steps / el128
-        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / c128_steps, "plan": "four-sweep",
+        result_c128 = {"value": v128, "ms_per_step": el128 * 1e3 / c128_steps, "plan": "separable",
+                       "hbm_frac": v128 * PASSES_PER_STEP * 2 * 16 / (HBM_PEAK_GBS * 1e9),
                        "speedup_c64_over_c128": value / v128, "mu": diag128["mu"]}
         if c128_steps == args.steps:
             acc = accuracy(psi0, x, final64, (solve128.to_numpy(), diag128))
         solve128.close()
-        log(f"c128 run: {c128_steps} steps, separable plan (the complex128 default)")
-        solve_s, el_s, _ = run_engine("c128", psi0, x, k, c128_steps, args.warmup, plan=_lib.PLAN_SEPARABLE)
-        solve_s.close()
-        vs = N ** 3 * c128_steps / el_s
-        result_c128["separable"] = {"value": vs, "ms_per_step": el_s * 1e3 / c128_steps, "sweeps_per_step": 3,
-                                    "hbm_frac": vs * 3 * 2 * 16 / (HBM_PEAK_GBS * 1e9),
-                                    "speedup_c64_over_it": value / vs}
+        log(f"four-sweep plan: c128 and c64, {c128_steps} steps each")
+        solve_f, el_f, _ = run_engine("c128", psi0, x, k, c128_steps, args.warmup, plan=_lib.PLAN_FOUR_SWEEP)
+        solve_f.close()
+        vf = N ** 3 * c128_steps / el_f
+        result_c128["four_sweep"] = {"value": vf, "ms_per_step": el_f * 1e3 / c128_steps, "sweeps_per_step": 4,
+                                     "speedup_c64_over_it": value / vf}
+        solve_f, el_f, _ = run_engine("c64", psi0, x, k, c128_steps, args.warmup, plan=_lib.PLAN_FOUR_SWEEP)
+        solve_f.close()
+        four64 = {"value": N ** 3 * c128_steps / el_f, "ms_per_step": el_f * 1e3 / c128_steps,
+                  "sweeps_per_step": 4, "hbm_frac_at_64B": N ** 3 * c128_steps / el_f * 64 / (HBM_PEAK_GBS * 1e9)}
         unfused_steps = min(c128_steps, args.unfused_steps)
         log(f"reference-structured c128 run: {unfused_steps} steps")
         solve_u, el_u, diag_u = run_engine("c128", psi0, x, k, unfused_steps, args.warmup, unfused=True)
@@ -531,7 +533,7 @@ def main(argv=None):
         "data": "synthetic (seeded Thomas-Fermi + noise), resident in HBM",
         "config": {"workload": "stirring run 256^3, L=40, beta=4000, dt=2.5e-4, nk every 500 steps "
                                "(dk shells and the 128-bin view up to k_max)",
-                   "grid": N, "precision": "complex64 field, FP64 reductions/time"},
+                   "grid": N, "precision": "complex64 field, FP64 reductions/time", "step_plan": "separable"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "algorithmic_bytes": bytes_per_launch, "avg_us": dom_ms * 1e3,
@@ -543,8 +545,12 @@ def main(argv=None):
                      "traced_avg_us": dom_traced_us,
                      "traced_frac": (bytes_per_launch / (dom_traced_us * 1e-6) / 1e9 / HBM_PEAK_GBS
                                      if dom_traced_us else None)},
+        # at the bytes this plan moves (3 sweeps: 48 B per point), and at the
+        # 64 B per point of a four-sweep step
         "step_roofline_frac": value * step_bytes / (HBM_PEAK_GBS * 1e9),
         "step_bytes_per_point": step_bytes,
+        "step_roofline_frac_at_64B": value * PASSES_FOUR_SWEEP * 2 * 8 / (HBM_PEAK_GBS * 1e9),
+        "c64_four_sweep": four64,
         "stream_copy_gbs": copy_gbs,
         "per_kernel": per_kernel,
         "mu_c64": diag64["mu"],

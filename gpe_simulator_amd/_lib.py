@@ -31,10 +31,8 @@ PLAN_SEPARABLE, PLAN_FOUR_SWEEP = 0, 1
 
 def default_plan(precision, kind, masked=False):
     """The plan a new solve steps with (gpe_solve_create's rule): four-sweep
-    with a momentum cutoff and for complex64 real time, else separable."""
-    if masked or (int(precision) == C64 and int(kind) == REAL):
-        return PLAN_FOUR_SWEEP
-    return PLAN_SEPARABLE
+    with a momentum cutoff, else separable."""
+    return PLAN_FOUR_SWEEP if masked else PLAN_SEPARABLE
 
 EXPORTS = (
     "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",

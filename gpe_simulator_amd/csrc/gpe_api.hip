@@ -652,18 +652,9 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
   return GPE_OK;
 }
 
-// Default step plan.  The momentum cutoff is not separable.  complex64 real
-// time keeps the four-sweep plan: the separable plan rounds three small
-// rotations per point and step (one per axis) instead of one, and each
-// rounding of a rotation by a tiny angle leans outward (the larger component
-// stays put while the smaller one moves), so the norm drifts; measured over
-// the 10000-step 128^3 quench at dt = 5e-5: +2.5e-5 against -1.2e-6 with one
-// rotation.  Imaginary time (a real decay, renormalised every step) and
-// complex128 (rounding at 1e-16) take the separable plan.
-int default_plan(int prec, int kind, int masked) {
-  if (masked || (prec == GPE_C64 && kind == GPE_REAL)) return GPE_PLAN_FOUR_SWEEP;
-  return GPE_PLAN_SEPARABLE;
-}
+// Default step plan: separable, except with the momentum cutoff, which is
+// not separable.
+int default_plan(int, int, int masked) { return masked ? GPE_PLAN_FOUR_SWEEP : GPE_PLAN_SEPARABLE; }
 
 // The pass kind that writes an imaginary-time step's norm partials.
 int norm_kind(int plan) { return plan == GPE_PLAN_SEPARABLE ? K_XK_LAST : K_Z_LAST; }

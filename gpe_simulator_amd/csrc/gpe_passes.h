@@ -489,11 +489,13 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   }
   // The separable plan's line of the potential along the pass's axis (Z
   // kinds: pz(z), both slots summed for ZK_MID; XK_LAST: the block's pxy(x, y)
-  // column, every pencil of an X block sharing its y), staged in LDS for
-  // complex128, where the LDS budget allows it; complex64 reads it from the
-  // cache, which keeps its Z and X passes at four blocks per CU.
-  constexpr bool kStageLine = TF::kFast && sizeof(R) == 8 &&
-                              (KIND == K_ZK_FIRST || KIND == K_ZK_MID || KIND == K_XK_LAST);
+  // column, every pencil of an X block sharing its y), staged in LDS where
+  // that measured faster: every complex128 kind, and complex64 ZK_MID (65.8
+  // -> 57.2 us at 256^3 though its blocks per CU fall from 4 to 3; the
+  // complex64 imaginary-time kinds ran slower staged, at 3 blocks, and read
+  // the line from the cache).
+  constexpr bool kStageLine =
+      TF::kFast && (KIND == K_ZK_MID || (sizeof(R) == 8 && (KIND == K_ZK_FIRST || KIND == K_XK_LAST)));
   __shared__ R line_lds[kStageLine ? TF::N : 1];
   if constexpr (kStageLine) {
     if constexpr (KIND == K_XK_LAST) {
@@ -784,15 +786,27 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         half_steps();
       }
       TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
-      // the axis's factor, applied with one rounding per component
+      // The axis's factor.  Imaginary time scales by a real factor (hi + lo:
+      // one significant product, so one rounding).  Real time rotates: the
+      // product is formed in FP64 from the FP64 table and rounded once
+      // (cmul_d).  The hi + lo complex product rounds twice (the cross term,
+      // then the sum), and rounding a rotation by a tiny angle twice leans
+      // outward: the complex64 norm of the 10000-step 128^3 quench (dt =
+      // 5e-5) drifted +2.5e-5 that way, against -4e-7 rounded once, which
+      // is what the four-sweep plan's single 3D rotation gives.
       if (s.imag) {
 #pragma unroll
         for (int m = 0; m < M; ++m)
           if (TF::valid(t, m, n)) v[m] = scale_re(v[m], kin1_at(TF::pos(t, m)));
       } else {
 #pragma unroll
-        for (int m = 0; m < M; ++m)
-          if (TF::valid(t, m, n)) v[m] = cmul_c(v[m], kin1_at(TF::pos(t, m)));
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) {
+            const int f = freq_index(TF::pos(t, m), n);
+            v[m] = cmul_d(v[m], gload(s.kin1 + (f < 0 ? -f : f)));
+          }
+          if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
+        }
       }
       TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
       if constexpr (kLocalX) half_steps();

@@ -114,11 +114,8 @@ int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size
  * 3 sweeps of psi per step.  GPE_PLAN_FOUR_SWEEP: the 3D FFT, the kinetic
  * factor and the inverse 3D FFT in four passes (z, y, x, y; 5 in imaginary
  * time).  A momentum cutoff (kcut_enable) is not separable, so such solves
- * use, and may only use, the four-sweep plan.  Defaults: separable for
- * imaginary time and for complex128; four-sweep for complex64 real time,
- * where the three separately rounded small rotations per step make the norm
- * drift (rounding leans outward for tiny angles).  Batch members share a
- * plan. */
+ * use, and may only use, the four-sweep plan; every other solve starts on the
+ * separable plan.  Batch members share a plan. */
 #define GPE_PLAN_SEPARABLE 0
 #define GPE_PLAN_FOUR_SWEEP 1
 int gpe_solve_set_plan(gpe_solve* solve, int32_t plan);
