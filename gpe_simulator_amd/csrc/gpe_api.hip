@@ -234,7 +234,7 @@ struct gpe_solve {
   void* work = nullptr;
   void* tw = nullptr;
   void* kin[3] = {nullptr, nullptr, nullptr};
-  void* kin1 = nullptr;                   // separable plan: 1D factor by |frequency|, 1/n folded in
+  void* kin1 = nullptr;                   // separable plan: 1D factor by FFT index, 1/n folded in
   int plan = GPE_PLAN_SEPARABLE;          // step plan (GPE_PLAN_*)
   double* xa = nullptr;
   double* ka = nullptr;
@@ -431,6 +431,7 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     // reference time
     s->stamp_chained = false;
   }
+  if (s->kind == GPE_IMAG) flags |= F_KIN1_LDS;
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
@@ -1032,7 +1033,7 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   const int n = s->n;
   if ((r = dzalloc(&s->psi, s->n3 * s->esz, s->st)) || (r = dzalloc(&s->tw, n * 16, s->st)) ||
       (r = dzalloc(&s->kin[0], n * 16, s->st)) || (r = dzalloc(&s->kin[1], n * 16, s->st)) ||
-      (r = dzalloc(&s->kin[2], n * 16, s->st)) || (r = dzalloc(&s->kin1, (n / 2 + 1) * 16, s->st)) ||
+      (r = dzalloc(&s->kin[2], n * 16, s->st)) || (r = dzalloc(&s->kin1, n * 16, s->st)) ||
       (r = dzalloc(&s->xa, n * sizeof(double), s->st)) ||
       (r = dzalloc(&s->ka, n * sizeof(double), s->st)) || (r = dzalloc(&s->m_xy, (size_t)n * n * sizeof(double), s->st)) ||
       (r = dzalloc(&s->m_z, n * sizeof(double), s->st)) || (r = dzalloc(&s->w_xy, (size_t)n * n * s->rsz, s->st)) ||
@@ -1577,6 +1578,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 }
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
+  if (b->kind == GPE_IMAG) flags |= F_KIN1_LDS;
   PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
   if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));

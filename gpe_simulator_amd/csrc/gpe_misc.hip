@@ -35,8 +35,8 @@ __global__ void k_twiddle(cpx<R>* tw, int n) {
 
 // Kinetic factors per axis in FP64: exp(-i*coef*k^2*dt) (real time) or
 // exp(-coef*k^2*dt) (imaginary time).  The x factor also carries 1/n^3.  k1
-// (n/2 + 1 entries, by |frequency|: the k of entry f is karr[f]) is the
-// separable plan's 1D factor, which carries 1/n for its axis's inverse FFT.
+// (n entries, by FFT index like the others) is the separable plan's 1D
+// factor, which carries 1/n for its axis's inverse FFT.
 template <typename R>
 __global__ void k_kinetic(cpx<R>* kx, cpx<R>* ky, cpx<R>* kz, cpx<R>* k1, const double* karr, int n, double dt,
                           double coef, int imag, double inv_n3) {
@@ -59,7 +59,7 @@ __global__ void k_kinetic(cpx<R>* kx, cpx<R>* ky, cpx<R>* kz, cpx<R>* k1, const 
   ky[i] = e;
   kz[i] = e;
   kx[i] = cpx<R>{R(re * inv_n3), R(im * inv_n3)};
-  if (i <= n / 2) k1[i] = cpx<R>{R(re / n), R(im / n)};
+  k1[i] = cpx<R>{R(re / n), R(im / n)};
 }
 
 // ------------------------------------------------------ expression VM --

@@ -40,10 +40,11 @@ using TF = typename Shape<GPE_PREC, GPE_N, GPE_V>::TF;
 template <int AXIS, int KIND>
 static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEvent_t t0, hipEvent_t t1) {
   const dim3 grid((unsigned)tile_count<TF, AXIS>(n), 1, (unsigned)batch);
+  const unsigned dyn = kin1_lds_bytes<TF, KIND, Real>(a.flags);
   if (t0 || t1)
-    hipExtLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, t0, t1, 0, a);
+    hipExtLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), dyn, st, t0, t1, 0, a);
   else
-    hipLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_pass<TF, AXIS, KIND, Real>), grid, dim3(TF::THREADS), dyn, st, a);
 }
 
 static int axis_of(int kind) {

@@ -63,6 +63,7 @@ enum PassFlags : int {
   F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
   F_SPECTRUM = 16, // X_DIAG: also bin |F|^2 into the shells and the view
   F_STAMP = 32,    // record the launch's end on the device wall clock
+  F_KIN1_LDS = 64, // separable kinds, imaginary time: the launch's dynamic LDS holds the 1D factor table
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -75,7 +76,7 @@ struct SolveDev {
   const cpx<double>* kin_x;  // kinetic factors per axis, FP64 (1/n^3 folded into x)
   const cpx<double>* kin_y;
   const cpx<double>* kin_z;
-  const cpx<double>* kin1;   // 1D factor of one axis by |frequency| (n/2 + 1 entries, 1/n folded in), FP64
+  const cpx<double>* kin1;   // 1D factor of one axis by FFT index (n entries, 1/n folded in), FP64
   const double* karr;      // k axis, FP64
   const double* xarr;      // x axis, FP64
   const R* pxy;            // potential table over (x, y): [slot][n*n]
@@ -386,30 +387,68 @@ __device__ __forceinline__ int view_index(int p, int q, int r, int h, int nview,
 // Tile = B pencils; one block per tile.  (A multi-tile variant that prefetched
 // the next tile into registers measured slower at every size: the live
 // prefetch registers cost more occupancy than the overlap gained.)
+// The separable plan's kinds (FFT along one axis, its 1D factor, inverse FFT).
+__host__ __device__ constexpr bool is_kin1_kind(int kind) {
+  return kind == K_ZK_FIRST || kind == K_ZK_MID || kind == K_YK || kind == K_XK || kind == K_XK_LAST;
+}
+
+// Dynamic LDS bytes a pass launch carries: the 1D factor table of an
+// imaginary-time separable pass (F_KIN1_LDS), else none.
+template <class TF, int KIND, typename R>
+__host__ __device__ constexpr unsigned kin1_lds_bytes(int flags) {
+  return TF::kFast && is_kin1_kind(KIND) && (flags & F_KIN1_LDS) ? unsigned(TF::N / 2 + 1) * sizeof(cconst<R>) : 0u;
+}
+
+// A pencil's first element is base = ubase + lane: ubase, the tile's first
+// element, is the same for the whole block; lane is the pencil's offset in
+// the tile.
 template <class TF, int AXIS>
 struct Tile {
-  long long base;
+  long long base, ubase;
+  unsigned lane;
   int pi, pj, pk;  // pencil coordinates (the axis coordinate is unused)
   __device__ __forceinline__ Tile(int tile, int b, int n) : pi(0), pj(0), pk(0) {
     if constexpr (AXIS == 2) {
       const long long row = (long long)tile * TF::B + b;
       pi = int(row / n);
       pj = int(row % n);
-      base = row * n;
+      ubase = (long long)tile * TF::B * n;
+      lane = unsigned(b) * unsigned(n);
     } else {
       const int tiles_per_row = n / TF::B;
       const int outer = tile / tiles_per_row;
-      pk = (tile - outer * tiles_per_row) * TF::B + b;
+      const int pk0 = (tile - outer * tiles_per_row) * TF::B;
+      pk = pk0 + b;
+      lane = unsigned(b);
       if constexpr (AXIS == 1) {
         pi = outer;
-        base = (long long)pi * n * n + pk;
+        ubase = (long long)pi * n * n + pk0;
       } else {
         pj = outer;
-        base = (long long)pj * n + pk;
+        ubase = (long long)pj * n + pk0;
       }
     }
+    base = ubase + lane;
   }
 };
+
+// Element m of thread t's pencil in a fast-transform tile, addressed as the
+// block-uniform pointer tile + TP*m*stride plus the thread's 32-bit byte
+// offset, so the compiler forms each address as a scalar base and a vector
+// offset (no 64-bit vector add per access).  The offset (lane + t*stride
+// elements) stays far below 2^32 bytes for every fast side.
+// (The base passes through readfirstlane, which pins it to scalar registers:
+// without it the compiler folds the offsets together and forms a 64-bit
+// vector address per element.)
+template <class TF, typename T>
+__device__ __forceinline__ T* tile_elem(T* tile, unsigned off_bytes, int m, long long stride) {
+  using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+  const unsigned long long p = reinterpret_cast<unsigned long long>(tile + (long long)TF::TP * m * stride);
+  const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(p));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(p >> 32));
+  Byte* base = reinterpret_cast<Byte*>((static_cast<unsigned long long>(hi) << 32) | lo);
+  return reinterpret_cast<T*>(base + off_bytes);
+}
 
 template <class TF, int AXIS>
 __host__ __device__ __forceinline__ int tile_count(int n) {
@@ -510,24 +549,41 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       }
     }
   }
-  // the separable plan's 1D factor, by |frequency| (n/2 + 1 entries)
-  constexpr bool kKin1 =
-      KIND == K_ZK_FIRST || KIND == K_ZK_MID || KIND == K_YK || KIND == K_XK || KIND == K_XK_LAST;
-  // (staged from the FP64 table in the field's constant representation,
-  // float hi + lo for complex64, so the product is one rounding of a factor
-  // carried to ~2^-48 in float arithmetic alone)
+  // The separable plan's 1D factor (s.kin1, by FFT index).  Real time reads
+  // the FP64 table at each element's index (one vector offset and immediate
+  // offsets for all of a thread's elements).  Imaginary time stages entries
+  // 0..n/2, read by |frequency|, in the field's constant representation
+  // (float hi + lo for complex64, so the product is one rounding of a factor
+  // carried to ~2^-48 in float arithmetic alone), in dynamic LDS that only
+  // imaginary-time launches carry (F_KIN1_LDS): without the 2 KiB table a
+  // complex64 256^3 separable block fits 4 times per CU instead of 3.
+  constexpr bool kKin1 = is_kin1_kind(KIND);
   constexpr bool kStageKin1 = TF::kFast && kKin1;
-  __shared__ cconst<R> kin1_lds[kStageKin1 ? TF::N / 2 + 1 : 1];
-  if constexpr (kStageKin1) {
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  cconst<R>* kin1_lds = reinterpret_cast<cconst<R>*>(dyn_lds);
+  const bool kin1_staged = kStageKin1 && (a.flags & F_KIN1_LDS) != 0;
+  if (kin1_staged) {
     for (int e = threadIdx.x; e <= TF::N / 2; e += TF::THREADS) {
       const cpx<double> f = gload(s.kin1 + e);
       kin1_lds[e] = make_cconst<R>(f.x, f.y);
     }
   }
-  auto kin1_at = [&](int i) -> cconst<R> {
-    const int f = freq_index(i, n);
-    const int af = f < 0 ? -f : f;
-    if constexpr (kStageKin1) {
+  // |frequency| of element m of thread t's pencil.  In a fast transform the
+  // first M/2 elements of every thread sit below n/2 (pos = t + TP*m with
+  // TP*M/2 = n/2) and the rest at or above it, so |f| is pos or n - pos by m
+  // alone.
+  auto kin1_index = [&](int t_, int m) -> int {
+    const int pos = TF::pos(t_, m);
+    if constexpr (TF::kFast) {
+      static_assert(TF::M % 2 == 0, "a fast transform's thread holds M even elements");
+      return m < M / 2 ? pos : TF::N - pos;
+    } else {
+      const int f = freq_index(pos, n);
+      return f < 0 ? -f : f;
+    }
+  };
+  auto kin1_at = [&](int af) -> cconst<R> {
+    if (kin1_staged) {
       return kin1_lds[af];
     } else {
       const cpx<double> w = gload(s.kin1 + af);
@@ -581,7 +637,13 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   cpx<R> v[M];
   {
     const Tile<TF, AXIS> g(tile, b, n);
-    load_tile<TF>(v, src, g.base, stride, t, n);
+    if constexpr (TF::kFast) {
+      const unsigned off = (g.lane + unsigned(t) * unsigned(stride)) * unsigned(sizeof(cpx<R>));
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m] = gload(tile_elem<TF>(src + g.ubase, off, m, stride));
+    } else {
+      load_tile<TF>(v, src, g.base, stride, t, n);
+    }
   }
   {
     const Tile<TF, AXIS> g(tile, b, n);
@@ -797,14 +859,11 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       if (s.imag) {
 #pragma unroll
         for (int m = 0; m < M; ++m)
-          if (TF::valid(t, m, n)) v[m] = scale_re(v[m], kin1_at(TF::pos(t, m)));
+          if (TF::valid(t, m, n)) v[m] = scale_re(v[m], kin1_at(kin1_index(t, m)));
       } else {
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          if (TF::valid(t, m, n)) {
-            const int f = freq_index(TF::pos(t, m), n);
-            v[m] = cmul_d(v[m], gload(s.kin1 + (f < 0 ? -f : f)));
-          }
+          if (TF::valid(t, m, n)) v[m] = cmul_d(v[m], gload(s.kin1 + TF::pos(t, m)));
           if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
@@ -882,10 +941,16 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     }
 
     if (write) {
-      const long long base = g.base;
+      if constexpr (TF::kFast) {
+        const unsigned off = (g.lane + unsigned(t) * unsigned(stride)) * unsigned(sizeof(cpx<R>));
 #pragma unroll
-      for (int m = 0; m < M; ++m)
-        if (TF::valid(t, m, n)) gstore(dst + base + (long long)TF::pos(t, m) * stride, v[m]);
+        for (int m = 0; m < M; ++m) gstore(tile_elem<TF>(dst + g.ubase, off, m, stride), v[m]);
+      } else {
+        const long long base = g.base;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          if (TF::valid(t, m, n)) gstore(dst + base + (long long)TF::pos(t, m) * stride, v[m]);
+      }
     }
   }
 
