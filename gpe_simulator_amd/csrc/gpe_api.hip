@@ -431,7 +431,7 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     // reference time
     s->stamp_chained = false;
   }
-  if (s->kind == GPE_IMAG) flags |= F_KIN1_LDS;
+  flags |= s->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
@@ -1578,7 +1578,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 }
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
-  if (b->kind == GPE_IMAG) flags |= F_KIN1_LDS;
+  flags |= b->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
   PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
   if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));

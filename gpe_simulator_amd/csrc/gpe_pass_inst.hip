@@ -37,8 +37,37 @@ using TF = typename Shape<GPE_PREC, GPE_N, GPE_V>::TF;
 #define GPE_SUFFIX GPE_CAT(GPE_CAT(GPE_CAT(GPE_CAT(GPE_CAT(p, GPE_PREC), _n), GPE_N), _v), GPE_V)
 #define GPE_LAUNCHER GPE_CAT(launch_pass_, GPE_SUFFIX)
 
+// Blocks of a launch that fit one CU's LDS (160 KiB, allocated in 512-byte
+// units) with `dyn` bytes of dynamic LDS beside the kernel's static LDS.
+static int blocks_per_cu_by_lds(unsigned static_bytes, unsigned dyn) {
+  const unsigned need = (static_bytes + dyn + 511u) / 512u * 512u;
+  return need ? int((160u * 1024u) / need) : 1 << 20;
+}
+
+// F_KIN1_RT: a real-time separable pass stages its factor table in LDS when
+// the table costs no blocks per CU (the kernel's static LDS is read once).
 template <int AXIS, int KIND>
-static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a, hipEvent_t t0, hipEvent_t t1) {
+static int resolve_flags(int flags) {
+  if (!(flags & F_KIN1_RT)) return flags;
+  flags &= ~F_KIN1_RT;
+  if constexpr (TF::kFast && is_kin1_kind(KIND)) {
+    static const bool fits = [] {
+      hipFuncAttributes attr{};
+      if (hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(&k_pass<TF, AXIS, KIND, Real>)) != hipSuccess)
+        return false;
+      const unsigned s = unsigned(attr.sharedSizeBytes);
+      const unsigned dyn = unsigned(TF::N / 2 + 1) * sizeof(cconst<Real>);
+      return blocks_per_cu_by_lds(s, dyn) >= blocks_per_cu_by_lds(s, 0);
+    }();
+    if (fits) flags |= F_KIN1_LDS;
+  }
+  return flags;
+}
+
+template <int AXIS, int KIND>
+static void go(int n, int batch, hipStream_t st, const PassArgs<Real>& a_in, hipEvent_t t0, hipEvent_t t1) {
+  PassArgs<Real> a = a_in;
+  a.flags = resolve_flags<AXIS, KIND>(a.flags);
   const dim3 grid((unsigned)tile_count<TF, AXIS>(n), 1, (unsigned)batch);
   const unsigned dyn = kin1_lds_bytes<TF, KIND, Real>(a.flags);
   if (t0 || t1)

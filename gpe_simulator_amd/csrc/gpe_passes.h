@@ -63,7 +63,9 @@ enum PassFlags : int {
   F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
   F_SPECTRUM = 16, // X_DIAG: also bin |F|^2 into the shells and the view
   F_STAMP = 32,    // record the launch's end on the device wall clock
-  F_KIN1_LDS = 64, // separable kinds, imaginary time: the launch's dynamic LDS holds the 1D factor table
+  F_KIN1_LDS = 64, // separable kinds: the launch's dynamic LDS holds the 1D factor table
+  F_KIN1_RT = 128, // separable kinds, real time: stage the 1D factor table in LDS where the launcher
+                   // finds that it costs no occupancy (the launcher turns it into F_KIN1_LDS)
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -559,13 +561,27 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // complex64 256^3 separable block fits 4 times per CU instead of 3.
   constexpr bool kKin1 = is_kin1_kind(KIND);
   constexpr bool kStageKin1 = TF::kFast && kKin1;
+  // Real time stages the same entries as FP64 (16 bytes each, as the
+  // complex64 hi + lo pair), where that keeps the launch's blocks per CU
+  // (F_KIN1_LDS set by the launcher on a real-time pass): the factor table is
+  // symmetric in the frequency, bit for bit, so the entry at |f| is the entry
+  // at the FFT index, and the loads of a thread's 16 factors (four round
+  // trips to the cache between the transforms) become LDS reads.
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   cconst<R>* kin1_lds = reinterpret_cast<cconst<R>*>(dyn_lds);
+  cpx<double>* kin1_rt_lds = reinterpret_cast<cpx<double>*>(dyn_lds);
+  static_assert(sizeof(cconst<R>) == sizeof(cpx<double>), "both staged factor tables take 16 bytes an entry");
+  // (imaginary-time launches of a fast transform always carry the table;
+  // real-time ones when the launcher set F_KIN1_LDS)
   const bool kin1_staged = kStageKin1 && (a.flags & F_KIN1_LDS) != 0;
   if (kin1_staged) {
-    for (int e = threadIdx.x; e <= TF::N / 2; e += TF::THREADS) {
-      const cpx<double> f = gload(s.kin1 + e);
-      kin1_lds[e] = make_cconst<R>(f.x, f.y);
+    if (s.imag) {
+      for (int e = threadIdx.x; e <= TF::N / 2; e += TF::THREADS) {
+        const cpx<double> f = gload(s.kin1 + e);
+        kin1_lds[e] = make_cconst<R>(f.x, f.y);
+      }
+    } else {
+      for (int e = threadIdx.x; e <= TF::N / 2; e += TF::THREADS) kin1_rt_lds[e] = gload(s.kin1 + e);
     }
   }
   // |frequency| of element m of thread t's pencil.  In a fast transform the
@@ -583,7 +599,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     }
   };
   auto kin1_at = [&](int af) -> cconst<R> {
-    if (kin1_staged) {
+    if constexpr (kStageKin1) {
       return kin1_lds[af];
     } else {
       const cpx<double> w = gload(s.kin1 + af);
@@ -860,6 +876,12 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
 #pragma unroll
         for (int m = 0; m < M; ++m)
           if (TF::valid(t, m, n)) v[m] = scale_re(v[m], kin1_at(kin1_index(t, m)));
+      } else if (kin1_staged) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          if (TF::valid(t, m, n)) v[m] = cmul_d(v[m], kin1_rt_lds[kin1_index(t, m)]);
+          if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
+        }
       } else {
 #pragma unroll
         for (int m = 0; m < M; ++m) {
