@@ -311,17 +311,30 @@ __device__ __forceinline__ double rexp(double x) { return exp(x); }
 // e = (beta dt/2) |psi|^2 + ph.  `vim` is the static imaginary part.  IMAG and
 // VIM (an absorber is present) are compile-time choices: the pass picks the
 // instantiation once per tile, so no point evaluates a factor it discards.
+// The exponent of one local phase, (beta dt/2)|psi|^2 + ph, and of two merged
+// ones without an absorber, 2 (beta dt/2)|psi|^2 + ph.
+template <bool MERGED, typename R>
+__device__ __forceinline__ R phase_exponent(cpx<R> v, R ph, const SolveDev<R>& s) {
+  const R a2 = v.x * v.x + v.y * v.y;
+  if constexpr (MERGED) return (s.bhdt + s.bhdt) * a2 + ph;
+  else return s.bhdt * a2 + ph;
+}
+// v * exp(-i e) from (sin e, cos e).
+template <typename R>
+__device__ __forceinline__ cpx<R> rotate_by(cpx<R> v, R sn, R cs) {
+  return cpx<R>{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+}
+
 template <bool IMAG, bool VIM, typename R>
 __device__ __forceinline__ cpx<R> half_step(cpx<R> v, R ph, R vim, const SolveDev<R>& s) {
-  const R a2 = v.x * v.x + v.y * v.y;
-  const R e = s.bhdt * a2 + ph;
+  const R e = phase_exponent<false>(v, ph, s);
   if constexpr (IMAG) {
     const R f = rexp(-e);
     return cpx<R>{v.x * f, v.y * f};
   } else {
     R sn, cs;
     phase_sincos(e, &sn, &cs);
-    cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+    cpx<R> r = rotate_by(v, sn, cs);
     if constexpr (VIM) {
       const R d = rexp(vim * s.hdt);
       r = cpx<R>{r.x * d, r.y * d};
@@ -336,18 +349,18 @@ __device__ __forceinline__ cpx<R> half_step(cpx<R> v, R ph, R vim, const SolveDe
 // d^2 exp(-i(ea + eb)) with ea + eb = (beta dt/2)(|psi|^2 + |psi|^2 d^2) + ph.
 template <bool VIM, typename R>
 __device__ __forceinline__ cpx<R> two_half_steps(cpx<R> v, R ph, R vim, const SolveDev<R>& s) {
-  const R a2 = v.x * v.x + v.y * v.y;
   R d2 = R(1), e;
   if constexpr (VIM) {
+    const R a2 = v.x * v.x + v.y * v.y;
     const R d = rexp(vim * s.hdt);
     d2 = d * d;
     e = s.bhdt * (a2 + a2 * d2) + ph;
   } else {
-    e = (s.bhdt + s.bhdt) * a2 + ph;
+    e = phase_exponent<true>(v, ph, s);
   }
   R sn, cs;
   phase_sincos(e, &sn, &cs);
-  cpx<R> r{v.x * cs + v.y * sn, v.y * cs - v.x * sn};
+  cpx<R> r = rotate_by(v, sn, cs);
   if constexpr (VIM) r = cpx<R>{r.x * d2, r.y * d2};
   return r;
 }
@@ -727,42 +740,71 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         constexpr bool IMAG = decltype(imag_tag)::value;
         constexpr bool VIM = decltype(vim_tag)::value;
         constexpr bool VOL = decltype(vol_tag)::value;
+        // the potential's share of the phase at element m, times dt/2
+        auto point_phase = [&](int m) -> R {
+          const int k = TF::pos(t, m);
+          R pzk;
+          if constexpr (kStagePz) {
+            pzk = pz_lds[k];
+          } else {
+            R pz = gload(s.pz + (a.slot_a * s.pz_ss + k));
+            if constexpr (KIND == K_Z_MID) pz += gload(s.pz + (a.slot_b * s.pz_ss + k));
+            pzk = pz;
+          }
+          R ph = cxy + pzk;
+          if constexpr (VOL) {
+            R pv = R(0);
+            if (s.vre) pv = gload(s.vre + (rowoff + k));
+            if constexpr (KIND == K_Z_MID) pv += pv;
+            if (s.vdyn) {
+              pv += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
+              if constexpr (KIND == K_Z_MID) pv += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
+            }
+            ph += pv;
+          }
+          return ph * s.hdt;
+        };
+        auto keep = [&](int m, cpx<R> w) {
+          if constexpr (KIND == K_Z_LAST) {
+            if (norm) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+          }
+          v[m] = w;
+        };
+        if constexpr (!IMAG && !VIM) {
+          // every exponent first, then one branch per thread (as in the
+          // separable passes' local half-steps)
+          R e[M];
+          bool small = true;
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-          if (TF::valid(t, m, n)) {
-            const int k = TF::pos(t, m);
-            R pzk;
-            if constexpr (kStagePz) {
-              pzk = pz_lds[k];
-            } else {
-              R pz = gload(s.pz + (a.slot_a * s.pz_ss + k));
-              if constexpr (KIND == K_Z_MID) pz += gload(s.pz + (a.slot_b * s.pz_ss + k));
-              pzk = pz;
+          for (int m = 0; m < M; ++m) {
+            if (TF::valid(t, m, n)) {
+              e[m] = phase_exponent<KIND == K_Z_MID>(v[m], point_phase(m), s);
+              small = small && fabs(e[m]) < R(0.78);
             }
-            R ph = cxy + pzk;
-            if constexpr (VOL) {
-              R pv = R(0);
-              if (s.vre) pv = gload(s.vre + (rowoff + k));
-              if constexpr (KIND == K_Z_MID) pv += pv;
-              if (s.vdyn) {
-                pv += gload(s.vdyn + (a.slot_a * s.vdyn_ss + rowoff + k));
-                if constexpr (KIND == K_Z_MID) pv += gload(s.vdyn + (a.slot_b * s.vdyn_ss + rowoff + k));
+          }
+          auto finish = [&](auto fast_tag) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+              if (TF::valid(t, m, n)) {
+                R sn, cs;
+                if constexpr (decltype(fast_tag)::value) sincos_poly(e[m], &sn, &cs);
+                else phase_sincos(e[m], &sn, &cs);
+                keep(m, rotate_by(v[m], sn, cs));
               }
-              ph += pv;
             }
-            ph = ph * s.hdt;
-            R vi = R(0);
-            if constexpr (VIM) vi = gload(s.vim + (rowoff + k));
-            cpx<R> w;
-            if constexpr (KIND == K_Z_MID) {
-              w = two_half_steps<VIM>(v[m], ph, vi, s);
-            } else {
-              w = half_step<IMAG, VIM>(v[m], ph, vi, s);
+          };
+          if (small) finish(std::true_type{});
+          else finish(std::false_type{});
+        } else {
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (TF::valid(t, m, n)) {
+              const R ph = point_phase(m);
+              R vi = R(0);
+              if constexpr (VIM) vi = gload(s.vim + (rowoff + TF::pos(t, m)));
+              if constexpr (KIND == K_Z_MID) keep(m, two_half_steps<VIM>(v[m], ph, vi, s));
+              else keep(m, half_step<IMAG, VIM>(v[m], ph, vi, s));
             }
-            if constexpr (KIND == K_Z_LAST) {
-              if (norm) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
-            }
-            v[m] = w;
           }
         }
       };
@@ -801,41 +843,76 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         } else {
           pen = gload(s.pz + (a.slot_a * s.pz_ss + pk));
         }
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-          if (TF::valid(t, m, n)) {
-            const int idx = TF::pos(t, m);
-            R line;
-            if constexpr (kStageLine) {
-              line = line_lds[idx];
-            } else if constexpr (kLocalZ) {
-              line = gload(s.pz + (a.slot_a * s.pz_ss + idx));
-              if constexpr (kMerged) line += s.pz_ss == 0 ? line : gload(s.pz + (a.slot_b * s.pz_ss + idx));
-            } else {
-              line = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)idx * n + pj));
-            }
-            R ph = pen + line;
+        // the potential's share of the phase at element m, times dt/2
+        auto point_phase = [&](int m) -> R {
+          const int idx = TF::pos(t, m);
+          R line;
+          if constexpr (kStageLine) {
+            line = line_lds[idx];
+          } else if constexpr (kLocalZ) {
+            line = gload(s.pz + (a.slot_a * s.pz_ss + idx));
+            if constexpr (kMerged) line += s.pz_ss == 0 ? line : gload(s.pz + (a.slot_b * s.pz_ss + idx));
+          } else {
+            line = gload(s.pxy + (a.slot_a * s.pxy_ss + (long long)idx * n + pj));
+          }
+          R ph = pen + line;
+          if constexpr (VOL) {
             const long long vidx = g.base + (long long)idx * stride;
-            if constexpr (VOL) {
-              R pv = R(0);
-              if (s.vre) pv = gload(s.vre + vidx);
-              if constexpr (kMerged) pv += pv;
-              if (s.vdyn) {
-                pv += gload(s.vdyn + (a.slot_a * s.vdyn_ss + vidx));
-                if constexpr (kMerged) pv += gload(s.vdyn + (a.slot_b * s.vdyn_ss + vidx));
+            R pv = R(0);
+            if (s.vre) pv = gload(s.vre + vidx);
+            if constexpr (kMerged) pv += pv;
+            if (s.vdyn) {
+              pv += gload(s.vdyn + (a.slot_a * s.vdyn_ss + vidx));
+              if constexpr (kMerged) pv += gload(s.vdyn + (a.slot_b * s.vdyn_ss + vidx));
+            }
+            ph += pv;
+          }
+          return ph * s.hdt;
+        };
+        auto keep = [&](int m, cpx<R> w) {
+          if constexpr (kLocalX) {
+            if (norm) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+          }
+          v[m] = w;
+        };
+        if constexpr (!IMAG && !VIM) {
+          // Real time without an absorber: every exponent first, then one
+          // branch per thread for all of its points (the polynomials alone
+          // when every phase is below 0.78 rad, as split-step phases are),
+          // instead of a branch around each point's sincos.  Same arithmetic
+          // as half_step / two_half_steps, so the same bits.
+          R e[M];
+          bool small = true;
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (TF::valid(t, m, n)) {
+              e[m] = phase_exponent<kMerged>(v[m], point_phase(m), s);
+              small = small && fabs(e[m]) < R(0.78);
+            }
+          }
+          auto finish = [&](auto fast_tag) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+              if (TF::valid(t, m, n)) {
+                R sn, cs;
+                if constexpr (decltype(fast_tag)::value) sincos_poly(e[m], &sn, &cs);
+                else phase_sincos(e[m], &sn, &cs);
+                keep(m, rotate_by(v[m], sn, cs));
               }
-              ph += pv;
             }
-            ph = ph * s.hdt;
-            R vi = R(0);
-            if constexpr (VIM) vi = gload(s.vim + vidx);
-            cpx<R> w;
-            if constexpr (kMerged) w = two_half_steps<VIM>(v[m], ph, vi, s);
-            else w = half_step<IMAG, VIM>(v[m], ph, vi, s);
-            if constexpr (kLocalX) {
-              if (norm) acc[0] += double(w.x) * double(w.x) + double(w.y) * double(w.y);
+          };
+          if (small) finish(std::true_type{});
+          else finish(std::false_type{});
+        } else {
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            if (TF::valid(t, m, n)) {
+              const R ph = point_phase(m);
+              R vi = R(0);
+              if constexpr (VIM) vi = gload(s.vim + (g.base + (long long)TF::pos(t, m) * stride));
+              if constexpr (kMerged) keep(m, two_half_steps<VIM>(v[m], ph, vi, s));
+              else keep(m, half_step<IMAG, VIM>(v[m], ph, vi, s));
             }
-            v[m] = w;
           }
         }
       };
