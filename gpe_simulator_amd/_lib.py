@@ -47,6 +47,7 @@ EXPORTS = (
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
     "gpe_step_kernel_times", "gpe_transform", "gpe_transform_download", "gpe_monitor_images",
     "gpe_volume_download", "gpe_step_noalloc", "gpe_solve_set_plan", "gpe_solve_get_plan",
+    "gpe_set_jit", "gpe_jit_stats",
 )
 
 # gpe_monitor_images bits (GPE_MON_* in include/gpe_hip.h), in output order
@@ -81,6 +82,8 @@ def _declare(lib):
     vp, i32, i64, dp = c_void_p, c_int32, c_int64, POINTER(c_double)
     sig = {
         "gpe_version": (c_char_p, []),
+        "gpe_set_jit": (c_int, [i32]),
+        "gpe_jit_stats": (c_int, [POINTER(i64), POINTER(i64), POINTER(i64)]),
         "gpe_last_error": (c_char_p, []),
         "gpe_ctx_create": (c_int, [c_int, POINTER(vp)]),
         "gpe_ctx_destroy": (c_int, [vp]),
@@ -226,6 +229,20 @@ class Context:
 
 
 _default_ctx = None
+
+
+def set_jit(enable):
+    """Time-dependent programs registered from now on run as kernels compiled
+    at run time (True, the default) or through the on-device interpreter."""
+    check(lib().gpe_set_jit(1 if enable else 0))
+
+
+def jit_stats():
+    """{"compiles", "launches", "failures"} of the run-time compiled programs
+    in this process."""
+    c, l, f = c_int64(), c_int64(), c_int64()
+    check(lib().gpe_jit_stats(byref(c), byref(l), byref(f)))
+    return {"compiles": c.value, "launches": l.value, "failures": f.value}
 
 
 def default_context():

@@ -53,7 +53,8 @@ def variants():
 
 def _units():
     units = [(CSRC / "gpe_api.hip", BUILD / "gpe_api.o", []),
-             (CSRC / "gpe_misc.hip", BUILD / "gpe_misc.o", [])]
+             (CSRC / "gpe_misc.hip", BUILD / "gpe_misc.o", []),
+             (CSRC / "gpe_jit.hip", BUILD / "gpe_jit.o", [])]
     for prec, side, var in variants():
         units.append((CSRC / "gpe_pass_inst.hip", BUILD / f"gpe_pass_p{prec}_n{side}_v{var}.o",
                       [f"-DGPE_PREC={prec}", f"-DGPE_N={side}", f"-DGPE_V={var}"]))
@@ -77,7 +78,10 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     with futures.ThreadPoolExecutor(max_workers=jobs) as pool:
         objects = list(pool.map(_compile, units))
     tmp = LIBRARY.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)]
+    # hiprtc compiles the time-dependent potential programs at run time
+    rocm_lib = Path(hipcc()).resolve().parents[1] / "lib"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects),
+           f"-L{rocm_lib}", "-lhiprtc", f"-Wl,-rpath,{rocm_lib}"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{proc.stderr}")

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "gpe_hip.h"
+#include "gpe_jit.h"
 #include "gpe_misc.h"
 
 using namespace gpe;
@@ -48,7 +50,14 @@ struct DevProgram {
   double* consts = nullptr;
   int ncode = 0;
   bool active = false;
+  JitProgram* jit = nullptr;     // run-time compiled tables (time-dependent programs); null: the interpreter
+  double* hv = nullptr;          // the compiled program's hoisted values, kMaxSlots rows
 };
+constexpr int kMaxSlots = 258;   // time slots of one table evaluation: a chunk of at most 256 steps, plus one
+
+// Time-dependent programs registered from now on run compiled (1, the
+// default) or through the interpreter (0).
+std::atomic<int> g_jit_enabled{1};
 
 struct PassArgsRaw {
   const void* s;
@@ -503,8 +512,11 @@ int validate_program(const int32_t* code, int32_t ncode, int32_t nconst, int tar
 int upload_program(DevProgram& p, const int32_t* code, int32_t ncode, const double* consts, int32_t nconst) {
   dfree(p.code);
   dfree(p.consts);
+  dfree(p.hv);
   p.code = nullptr;
   p.consts = nullptr;
+  p.hv = nullptr;
+  p.jit = nullptr;
   CHK(dmalloc((void**)&p.code, sizeof(int) * ncode));
   CHK(dmalloc((void**)&p.consts, sizeof(double) * std::max(1, nconst)));
   HIPCHK(hipMemcpy(p.code, code, sizeof(int) * ncode, hipMemcpyHostToDevice));
@@ -584,6 +596,11 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream
     const long long size = target_size(s, target);
     const double* master = g == 0 ? s->m_xy : (g == 1 ? s->m_z : nullptr);
     void* out = diag ? s->g_tab[g] : s->d_tab[g];
+    if (s->dyn[g].jit) {
+      HIPCHK(jit_eval(s->dyn[g].jit, s->prec == GPE_C64, s->dyn[g].consts, target, s->n, s->xa, size, master, dst_t,
+                      nslots, s->dyn[g].hv, out, st));
+      continue;
+    }
     if (s->prec == GPE_C64)
       launch_vm_dynamic<float>(s->dyn[g].code, s->dyn[g].ncode, s->dyn[g].consts, target, s->n, s->xa, size,
                                master, dst_t, nslots, out, st);
@@ -840,6 +857,20 @@ const char* gpe_version(void) { return "gpe_hip 0.1 (gfx950)"; }
 
 const char* gpe_last_error(void) { return g_err.c_str(); }
 
+int gpe_set_jit(int32_t enable) {
+  g_jit_enabled.store(enable ? 1 : 0);
+  return GPE_OK;
+}
+
+int gpe_jit_stats(int64_t* compiles, int64_t* launches, int64_t* failures) {
+  long long c = 0, l = 0, f = 0;
+  jit_stats(&c, &l, &f);
+  if (compiles) *compiles = c;
+  if (launches) *launches = l;
+  if (failures) *failures = f;
+  return GPE_OK;
+}
+
 int gpe_ctx_create(int device, gpe_ctx** out) {
   if (!out) return fail(GPE_ERR_INVALID, "out is null");
   int count = 0;
@@ -1074,6 +1105,7 @@ int gpe_solve_destroy(gpe_solve* s) {
   for (auto& p : s->dyn) {
     dfree(p.code);
     dfree(p.consts);
+    dfree(p.hv);
   }
   for (int f = 0; f < 2; ++f) {
     if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
@@ -1209,7 +1241,17 @@ int gpe_potential_dynamic(gpe_solve* s, int32_t target, const int32_t* code, int
   CHK(validate_program(code, ncode, nconst, target, true));
   CHK(set_device(s));
   HIPCHK(hipStreamSynchronize(s->st));
-  CHK(upload_program(s->dyn[target], code, ncode, consts, nconst));
+  DevProgram& p = s->dyn[target];
+  CHK(upload_program(p, code, ncode, consts, nconst));
+  if (g_jit_enabled.load()) {
+    p.jit = jit_get(code, ncode, nconst);
+    if (p.jit && jit_hoisted(p.jit) > 0) {
+      if (int r = dmalloc((void**)&p.hv, sizeof(double) * kMaxSlots * jit_hoisted(p.jit))) {
+        p.jit = nullptr;
+        return r;
+      }
+    }
+  }
   return layout_dynamic(s);
 }
 

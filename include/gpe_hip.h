@@ -150,6 +150,17 @@ int gpe_potential_static(gpe_solve* solve, int32_t target, const int32_t* code, 
                          const double* consts, int32_t nconst, double scale);
 int gpe_potential_dynamic(gpe_solve* solve, int32_t target, const int32_t* code, int32_t ncode,
                           const double* consts, int32_t nconst);
+/* Time-dependent programs are evaluated by kernels compiled at run time
+ * (hiprtc) from the program's structure, as the reference compiles its
+ * spliced expressions at run time (CuPy RawKernel, gpe/make_kernel.py:4-84);
+ * programs that differ only in their constants share one compiled module,
+ * and the tables are bit-identical to the interpreter's.  gpe_set_jit(0)
+ * makes programs registered afterwards use the on-device interpreter (the
+ * path taken anyway when a compile fails); gpe_jit_stats reports modules
+ * compiled, table evaluations launched and failed compiles, process-wide.
+ * No reference counterpart: CuPy's kernel cache plays this part there. */
+int gpe_set_jit(int32_t enable);
+int gpe_jit_stats(int64_t* compiles, int64_t* launches, int64_t* failures);
 /* Static volume V_static = re + i*im (target GPE_TGT_VOL / GPE_TGT_VOL_IM):
  * fill with a constant, or upload a slab of host float64 values that replace
  * (accumulate = 0) or are added to (accumulate = 1) elements [offset,
