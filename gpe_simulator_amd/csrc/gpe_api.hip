@@ -41,7 +41,10 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kPassKinds = 16;
-constexpr long long kDynBudget = 64LL << 20;   // bytes of time-dependent slot tables
+// Bytes of time-dependent slot tables per program: a 256^3 complex64 volume
+// program (64 MiB a slot) evaluates 16 slots per 15 steps within it; with 64
+// MiB it evaluated 2 slots every step, half of them again the next step.
+constexpr long long kDynBudget = 1LL << 30;
 constexpr long long kStageElems = 1LL << 20;   // complex128 elements per staging copy
 const double kUnitScale = 1.0;                  // host source of unit pending scales
 
