@@ -544,12 +544,34 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // The separable plan's line of the potential along the pass's axis (Z
   // kinds: pz(z), both slots summed for ZK_MID; XK_LAST: the block's pxy(x, y)
   // column, every pencil of an X block sharing its y), staged in LDS where
-  // that measured faster: every complex128 kind, and complex64 ZK_MID (65.8
-  // -> 57.2 us at 256^3 though its blocks per CU fall from 4 to 3; the
-  // complex64 imaginary-time kinds ran slower staged, at 3 blocks, and read
-  // the line from the cache).
+  // that measured faster: every complex128 kind, and complex64 ZK_MID at the
+  // sides below (65.8 -> 57.2 us at 256^3 when it was staged there, though
+  // its blocks per CU fell from 4 to 3; the complex64 imaginary-time kinds
+  // ran slower staged, at 3 blocks, and read the line from the cache).
+  //
+  // Complex64 ZK_FIRST / ZK_MID at sides that are multiples of 64 up to 256
+  // hold the z line in registers instead: lane L of every wave keeps entries
+  // L + 64 j (N / 64 registers), and a point fetches its entry from the lane
+  // that holds it (ds_bpermute: the LDS crossbar, no LDS allocation).  A
+  // wave's pencils all run along z and share the line, and every lane of the
+  // block is active (a block is one whole tile).  The 1 KiB this frees lets
+  // the real-time 1D factor table into LDS at 4 blocks per CU.
+  constexpr bool kRegLine = TF::kFast && sizeof(R) == 4 && (KIND == K_ZK_FIRST || KIND == K_ZK_MID) &&
+                            64 % TF::TP == 0 && TF::N % 64 == 0 && TF::N <= 256;
+  constexpr int kRegLineN = kRegLine ? TF::N / 64 : 1;
+  R line_reg[kRegLineN];
+  if constexpr (kRegLine) {
+    const int lane = int(threadIdx.x % 64);
+#pragma unroll
+    for (int j = 0; j < kRegLineN; ++j) {
+      R pz = gload(s.pz + (a.slot_a * s.pz_ss + lane + 64 * j));
+      if constexpr (KIND == K_ZK_MID) pz += gload(s.pz + (a.slot_b * s.pz_ss + lane + 64 * j));
+      line_reg[j] = pz;
+    }
+  }
   constexpr bool kStageLine =
-      TF::kFast && (KIND == K_ZK_MID || (sizeof(R) == 8 && (KIND == K_ZK_FIRST || KIND == K_XK_LAST)));
+      TF::kFast && !kRegLine &&
+      (KIND == K_ZK_MID || (sizeof(R) == 8 && (KIND == K_ZK_FIRST || KIND == K_XK_LAST)));
   __shared__ R line_lds[kStageLine ? TF::N : 1];
   if constexpr (kStageLine) {
     if constexpr (KIND == K_XK_LAST) {
@@ -847,7 +869,12 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         auto point_phase = [&](int m) -> R {
           const int idx = TF::pos(t, m);
           R line;
-          if constexpr (kStageLine) {
+          if constexpr (kRegLine) {
+            // idx = t + TP m = (t + TP (m % per)) + 64 (m / per)
+            constexpr int per = 64 / TF::TP;
+            const int src_lane = t + TF::TP * (m % per);
+            line = __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane * 4, __float_as_int(line_reg[m / per])));
+          } else if constexpr (kStageLine) {
             line = line_lds[idx];
           } else if constexpr (kLocalZ) {
             line = gload(s.pz + (a.slot_a * s.pz_ss + idx));
