@@ -23,16 +23,27 @@ REAL, IMAG = 0, 1
 TGT_XY, TGT_Z, TGT_VOL, TGT_VOL_IM = 0, 1, 2, 3
 
 PASS_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin", "z_diag", "x_diag", "z_fwd", "y_lz", "x_lz",
-              "zk_first", "zk_mid", "yk", "xk", "xk_last")
+              "zk_first", "zk_mid", "yk", "xk", "xk_last", "zyk_first", "zyk_mid")
 # step plans (GPE_PLAN_*): three sweeps per step with the kinetic factor split
-# per axis, or the four-sweep 3D-transform plan (the only one with a cutoff)
-PLAN_SEPARABLE, PLAN_FOUR_SWEEP = 0, 1
+# per axis, the four-sweep 3D-transform plan (the only one with a cutoff), or
+# the separable plan in two sweeps with the z and y axes of each x plane in one
+# pass (sides whose plane fits the LDS)
+PLAN_SEPARABLE, PLAN_FOUR_SWEEP, PLAN_PLANE = 0, 1, 2
 
 
-def default_plan(precision, kind, masked=False):
+# sides with a plane pass (an x plane of the field fits the LDS) per precision
+PLANE_SIDES = {0: (64, 96, 128), 1: (64,)}
+
+
+def default_plan(precision, kind, masked=False, n=None):
     """The plan a new solve steps with (gpe_solve_create's rule): four-sweep
-    with a momentum cutoff, else separable."""
-    return PLAN_FOUR_SWEEP if masked else PLAN_SEPARABLE
+    with a momentum cutoff; the plane plan at the plane sides that are
+    multiples of 64 (faster alone and in batches); else separable."""
+    if masked:
+        return PLAN_FOUR_SWEEP
+    if n is not None and n % 64 == 0 and n in PLANE_SIDES[int(precision)]:
+        return PLAN_PLANE
+    return PLAN_SEPARABLE
 
 EXPORTS = (
     "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",
@@ -423,7 +434,7 @@ class Solve:
         return {k: (float(us[i]), int(cnt[i])) for i, k in enumerate(PASS_KINDS)}
 
     def set_plan(self, plan):
-        """Step plan: PLAN_SEPARABLE (default) or PLAN_FOUR_SWEEP."""
+        """Step plan: PLAN_SEPARABLE (default), PLAN_FOUR_SWEEP or PLAN_PLANE."""
         check(lib().gpe_solve_set_plan(self.handle, int(plan)))
 
     @property

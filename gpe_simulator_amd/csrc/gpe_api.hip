@@ -40,7 +40,7 @@ int fail(int code, const std::string& msg) {
     if (r_ != GPE_OK) return r_;   \
   } while (0)
 
-constexpr int kPassKinds = 16;
+constexpr int kPassKinds = 18;   // GPE_PASS_KINDS: the kinds a step uses (timed and stamped)
 // Bytes of time-dependent slot tables per program: a 256^3 complex64 volume
 // program (64 MiB a slot) evaluates 16 slots per 15 steps within it; with 64
 // MiB it evaluated 2 slots every step, half of them again the next step.
@@ -236,10 +236,14 @@ struct gpe_solve {
   const PassVariant* vt = nullptr;        // compiled variants of this (precision, side)
   int nvt = 0;
   int var[kPassKinds] = {0};              // variant used by each pass kind
-  long long blocks(int kind, int n) const { return vt[var[tuned_kind(kind)]].blocks(kind, n); }
+  // (plane passes are compiled in the first variant's unit of their side)
+  const PassVariant& variant_of(int kind) const {
+    return kind == K_ZYK_FIRST || kind == K_ZYK_MID ? vt[0] : vt[var[tuned_kind(kind)]];
+  }
+  long long blocks(int kind, int n) const { return variant_of(kind).blocks(kind, n); }
   hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) const {
-    return vt[var[tuned_kind(kind)]].launch(kind, args, n, batch, stream, t0, t1);
+    return variant_of(kind).launch(kind, args, n, batch, stream, t0, t1);
   }
 
   void* slab = nullptr;                   // psi, tables and descriptors (one allocation)
@@ -438,16 +442,17 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     // the norm finisher follows an imaginary Z_LAST on the stream
     s->stamp_chained = !(kind == K_Z_LAST && (flags & F_NORM));
     flags |= F_STAMP;
-  } else if (s->stamping && kind < kPassKinds) {
-    // out of slots: this launch goes untimed, and a later (smaller) launch
-    // that still fits must not count it, so its span starts at a new
-    // reference time
+  } else if (s->stamping) {
+    // out of slots, or a kind outside the step kinds: this launch goes
+    // untimed, and a later launch must not count it, so its span starts at a
+    // new reference time
     s->stamp_chained = false;
   }
   flags |= s->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
+  else if (s->timing) s->timer.chained = false;   // an untimed kind: the next span starts after it
   hipError_t e = s->launch(kind, &a, s->n, 1, s->st, start, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString(e));
   return GPE_OK;
@@ -624,6 +629,10 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream
 // Separable plan (GPE_PLAN_SEPARABLE), 3 sweeps per step:
 //   real:      [ZK_FIRST(0)] YK  XK ZK_MID(q, q+1)  YK  XK ...  YK XK_LAST(n-1)
 //   imaginary: ZK_FIRST(prescale) YK XK_LAST(norm)     per step
+// Plane plan (GPE_PLAN_PLANE): the separable plan with each Z kind and the YK
+// after it in one plane pass, 2 sweeps per step:
+//   real:      [ZYK_FIRST(0)] XK ZYK_MID(q, q+1)  XK ...  XK_LAST(n-1)
+//   imaginary: ZYK_FIRST(prescale) XK_LAST(norm)     per step
 // Four-sweep plan (GPE_PLAN_FOUR_SWEEP, and masked solves):
 //   real:      [Z_FIRST(0)] Y_FWD X_KIN Y_INV Z_MID(q, q+1) ... Z_LAST(n-1)
 //   imaginary: Z_FIRST(prescale) Y_FWD X_KIN Y_INV Z_LAST(norm)
@@ -634,12 +643,21 @@ template <class P>
 int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bool prescale_first, bool dyn,
                P&& pass_fn) {
   const bool sep = plan == GPE_PLAN_SEPARABLE;
+  const bool plane = plan == GPE_PLAN_PLANE;
   for (int64_t q = q0; q < q0 + count; ++q) {
     const int sa = dyn ? (int)(q - q0) : 0, sb = dyn ? sa + 1 : 0;
     const bool first = q == 0, last = q == nsteps - 1;
     if (kind == GPE_REAL) {
       const int pre = first && prescale_first ? F_PRESCALE : 0;
-      if (sep) {
+      if (plane) {
+        if (first) CHK(pass_fn(K_ZYK_FIRST, sa, 0, pre, 0));
+        if (last) {
+          CHK(pass_fn(K_XK_LAST, sa, 0, 0, 0));
+        } else {
+          CHK(pass_fn(K_XK, 0, 0, 0, 0));
+          CHK(pass_fn(K_ZYK_MID, sa, sb, 0, 0));
+        }
+      } else if (sep) {
         if (first) CHK(pass_fn(K_ZK_FIRST, sa, 0, pre, 0));
         CHK(pass_fn(K_YK, 0, 0, 0, 0));
         if (last) {
@@ -658,7 +676,10 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
       }
     } else {
       const int pre = (first ? prescale_first : true) ? F_PRESCALE : 0;
-      if (sep) {
+      if (plane) {
+        CHK(pass_fn(K_ZYK_FIRST, 0, 0, pre, 0));
+        CHK(pass_fn(K_XK_LAST, 0, 0, F_NORM, (int)q));
+      } else if (sep) {
         CHK(pass_fn(K_ZK_FIRST, 0, 0, pre, 0));
         CHK(pass_fn(K_YK, 0, 0, 0, 0));
         CHK(pass_fn(K_XK_LAST, 0, 0, F_NORM, (int)q));
@@ -674,12 +695,21 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
   return GPE_OK;
 }
 
-// Default step plan: separable, except with the momentum cutoff, which is
-// not separable.
-int default_plan(int, int, int masked) { return masked ? GPE_PLAN_FOUR_SWEEP : GPE_PLAN_SEPARABLE; }
+// Default step plan: four-sweep with the momentum cutoff, which is not
+// separable; else the plane plan where a plane pass exists and its blocks are
+// whole waves of 8-thread pencils (complex64 at 64 and 128, complex128 at 64:
+// faster alone and in batches); else separable.  At 96 (4 threads a pencil,
+// 24 values each, one 6-wave block per CU) the plane pass is faster alone but
+// slower in batches, so solves there keep the separable plan unless set.
+// (_lib.default_plan restates this rule.)
+int default_plan(const gpe_solve* s) {
+  if (s->masked) return GPE_PLAN_FOUR_SWEEP;
+  if (s->n % 64 == 0 && s->blocks(K_ZYK_MID, s->n) > 0) return GPE_PLAN_PLANE;
+  return GPE_PLAN_SEPARABLE;
+}
 
 // The pass kind that writes an imaginary-time step's norm partials.
-int norm_kind(int plan) { return plan == GPE_PLAN_SEPARABLE ? K_XK_LAST : K_Z_LAST; }
+int norm_kind(int plan) { return plan == GPE_PLAN_FOUR_SWEEP ? K_Z_LAST : K_XK_LAST; }
 
 int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
   const bool pre = s->pending;
@@ -978,9 +1008,12 @@ int gpe_solve_get_variants(gpe_solve* s, int32_t* variants, int32_t nkinds) {
 
 int gpe_solve_set_plan(gpe_solve* s, int32_t plan) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
-  if (plan != GPE_PLAN_SEPARABLE && plan != GPE_PLAN_FOUR_SWEEP) return fail(GPE_ERR_INVALID, "unknown step plan");
-  if (plan == GPE_PLAN_SEPARABLE && s->masked)
+  if (plan != GPE_PLAN_SEPARABLE && plan != GPE_PLAN_FOUR_SWEEP && plan != GPE_PLAN_PLANE)
+    return fail(GPE_ERR_INVALID, "unknown step plan");
+  if (plan != GPE_PLAN_FOUR_SWEEP && s->masked)
     return fail(GPE_ERR_INVALID, "the momentum cutoff is not separable: a masked solve steps with the four-sweep plan");
+  if (plan == GPE_PLAN_PLANE && s->blocks(K_ZYK_MID, s->n) == 0)
+    return fail(GPE_ERR_INVALID, "no plane pass at this side and precision (a plane of the field must fit the LDS)");
   CHK(set_device(s));
   HIPCHK(hipStreamSynchronize(s->st));
   s->plan = plan;
@@ -1096,7 +1129,7 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
   const int imag = s->kind == GPE_IMAG ? 1 : 0;
   launch_twiddle<double>(s->tw, n, s->st);
   launch_kinetic<double>(s->kin[0], s->kin[1], s->kin[2], s->kin1, s->ka, n, s->dt, s->coef, imag, inv_n3, s->st);
-  s->plan = default_plan(s->prec, s->kind, s->masked);
+  s->plan = default_plan(s);
   if (hipGetLastError() != hipSuccess) return bail(fail(GPE_ERR_HIP, "table kernels failed to launch"));
   if ((r = ensure_red(s, std::max<long long>(kNormBlocks, std::max(s->blocks(K_Z_LAST, n), s->blocks(K_XK_LAST, n))))))
     return bail(r);
@@ -1637,7 +1670,8 @@ int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   flags |= b->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
   PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
-  if (b->timing) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
+  if (b->timing && kind < kPassKinds) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
+  else if (b->timing) b->timer.chained = false;   // an untimed kind: the next span starts after it
   hipError_t e = b->solves[0]->launch(kind, &a, b->n, (int)b->solves.size(), b->st, start, stop);
   if (e != hipSuccess) return fail(GPE_ERR_HIP, std::string("batched pass launch: ") + hipGetErrorString(e));
   return GPE_OK;

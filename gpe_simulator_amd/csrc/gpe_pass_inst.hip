@@ -6,6 +6,7 @@
 #include <hip/hip_ext.h>
 
 #include "gpe_passes.h"
+#include "gpe_plane.h"
 #include "gpe_shapes.h"
 
 #ifndef GPE_PREC
@@ -84,6 +85,30 @@ static int axis_of(int kind) {
   }
 }
 
+// Plane passes (one block per x plane) are compiled once per (precision,
+// side), in the first variant's unit, where the plane fits the LDS.
+template <class T>
+static constexpr bool has_plane() {
+  if constexpr (T::kFast) {
+    return GPE_V == 0 && 64 % T::TP == 0 && T::TP * T::N <= 1024 &&
+           plane_fits(plane_lds_bytes<Real, T::N, T::TP>());
+  } else {
+    return false;
+  }
+}
+
+template <int KIND, class T = TF>
+static void go_plane(int batch, hipStream_t st, const PassArgs<Real>& a, hipEvent_t t0, hipEvent_t t1) {
+  if constexpr (has_plane<T>()) {
+    constexpr int N = T::N, TP = T::TP;
+    const dim3 grid((unsigned)N, 1, (unsigned)batch);
+    if (t0 || t1)
+      hipExtLaunchKernelGGL((k_plane<Real, N, TP, KIND>), grid, dim3(TP * N), 0, st, t0, t1, 0, a);
+    else
+      hipLaunchKernelGGL((k_plane<Real, N, TP, KIND>), grid, dim3(TP * N), 0, st, a);
+  }
+}
+
 // A pass that writes FP64 norm partials, followed by k_norm_finish (one block
 // per solve) on the same stream; the finisher carries the end event.
 template <int AXIS, int KIND>
@@ -131,13 +156,23 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
     case K_Z_INV: go<2, K_Z_INV>(n, batch, st, a, t0, t1); break;
     case K_X_FWD: go<0, K_X_FWD>(n, batch, st, a, t0, t1); break;
     case K_X_INV: go<0, K_X_INV>(n, batch, st, a, t0, t1); break;
+    case K_ZYK_FIRST:
+      if (!has_plane<TF>()) return hipErrorInvalidValue;
+      go_plane<K_ZYK_FIRST>(batch, st, a, t0, t1);
+      break;
+    case K_ZYK_MID:
+      if (!has_plane<TF>()) return hipErrorInvalidValue;
+      go_plane<K_ZYK_MID>(batch, st, a, t0, t1);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// Blocks per solve that a pass of `kind` launches (sizes partial buffers).
+// Blocks per solve that a pass of `kind` launches (sizes partial buffers);
+// 0 for a plane kind that this unit does not compile.
 long long GPE_CAT(pass_blocks_, GPE_SUFFIX)(int kind, int n) {
+  if (kind == K_ZYK_FIRST || kind == K_ZYK_MID) return has_plane<TF>() ? n : 0;
   const int axis = axis_of(kind);
   return axis == 2 ? tile_count<TF, 2>(n) : (axis == 1 ? tile_count<TF, 1>(n) : tile_count<TF, 0>(n));
 }

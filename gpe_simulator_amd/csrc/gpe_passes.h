@@ -48,12 +48,17 @@ enum PassKind : int {
   K_YK = 13,       // FFT_y -> ky factor -> IFFT_y
   K_XK = 14,       // FFT_x -> kx factor -> IFFT_x
   K_XK_LAST = 15,  // FFT_x -> kx factor -> IFFT_x -> half(slot_a) [-> FP64 norm partials]
+  // Plane passes (gpe_plane.h): one block per x plane runs a Z kind of the
+  // separable plan and then YK, so a step sweeps psi twice instead of three
+  // times where a plane fits the LDS.
+  K_ZYK_FIRST = 16, // ZK_FIRST then YK
+  K_ZYK_MID = 17,   // ZK_MID then YK   (real time)
   // Plain steps of the reference-structured (unfused) real-time step: one
   // launch per reference kernel and transform axis, for measurement.
-  K_Z_HALF = 16,   // half(slot_a) alone, in real space
-  K_Z_INV = 17,    // plain IFFT_z
-  K_X_FWD = 18,    // plain FFT_x
-  K_X_INV = 19,    // plain IFFT_x
+  K_Z_HALF = 18,   // half(slot_a) alone, in real space
+  K_Z_INV = 19,    // plain IFFT_z
+  K_X_FWD = 20,    // plain FFT_x
+  K_X_INV = 21,    // plain IFFT_x
 };
 
 enum PassFlags : int {

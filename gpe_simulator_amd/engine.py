@@ -35,7 +35,7 @@ def make_solve(n, x_array, k_array, kind, dt, beta, static=None, dynamic=None, k
     ctx = ctx or _lib.default_context()
     # pass variants: tuned once per context, shape and step plan, then the
     # context default
-    plan_code = _lib.default_plan(precision_code(precision), kind_code, masked=kcut is not None)
+    plan_code = _lib.default_plan(precision_code(precision), kind_code, masked=kcut is not None, n=n)
     tune.ensure_tuned(n, precision_code(precision), kind_code, ctx, plan=plan_code)
     solve = _lib.Solve(n, precision_code(precision), kind_code, dt, beta, x_array, k_array, kcut=kcut, ctx=ctx)
     plan = plan_potential(static, dynamic if kind_code == _lib.REAL else None)

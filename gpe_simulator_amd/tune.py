@@ -38,14 +38,18 @@ from . import _lib
 from .grids import get_grids
 
 STEP_KINDS = {_lib.PLAN_SEPARABLE: ("zk_first", "zk_mid", "yk", "xk", "xk_last"),
-              _lib.PLAN_FOUR_SWEEP: ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin")}
+              _lib.PLAN_FOUR_SWEEP: ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin"),
+              # (the plane passes have one shape per side; only the x passes vary)
+              _lib.PLAN_PLANE: ("xk", "xk_last")}
 # pass kinds of a step in the order the descent visits them (most time first)
 STEP_SEQUENCE = {
     _lib.PLAN_SEPARABLE: {_lib.REAL: ("zk_mid", "xk", "yk", "zk_first", "xk_last"),
                           _lib.IMAG: ("xk_last", "zk_first", "yk")},
     _lib.PLAN_FOUR_SWEEP: {_lib.REAL: ("z_mid", "x_kin", "y_fwd", "y_inv", "z_first", "z_last"),
                            _lib.IMAG: ("z_first", "z_last", "x_kin", "y_fwd", "y_inv")},
+    _lib.PLAN_PLANE: {_lib.REAL: ("xk", "xk_last"), _lib.IMAG: ("xk_last",)},
 }
+PLAN_NAMES = {_lib.PLAN_SEPARABLE: "separable", _lib.PLAN_FOUR_SWEEP: "four-sweep", _lib.PLAN_PLANE: "plane"}
 GAIN = 0.005        # a descent move must shorten the step by more than this fraction
 _lock = threading.Lock()
 
@@ -68,7 +72,7 @@ class Tuning:
         kinds = STEP_KINDS[self.plan]
         lines = [f"n={self.n} precision={'c64' if self.precision == _lib.C64 else 'c128'} "
                  f"kind={'imaginary' if self.kind == _lib.IMAG else 'real'} batch={self.batch} "
-                 f"plan={'separable' if self.plan == _lib.PLAN_SEPARABLE else 'four-sweep'}"]
+                 f"plan={PLAN_NAMES[self.plan]}"]
         for v, geo in enumerate(self.variants):
             t = self.timings.get(v, {})
             lines.append(f"  v{v} tp={geo[0]} b={geo[1]} waves={geo[2]}: " +
@@ -205,7 +209,7 @@ def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None, plan=Non
     Tuning).  Kinds outside the plan keep `start` (default: variant 0)."""
     precision = int(precision)
     kind = int(kind)
-    plan = _lib.default_plan(precision, kind) if plan is None else int(plan)
+    plan = _lib.default_plan(precision, kind, n=n) if plan is None else int(plan)
     ctx = ctx or _lib.default_context()
     variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx, plan=plan)
     event_choice = list(start) if start is not None else [0] * len(_lib.PASS_KINDS)
@@ -233,8 +237,8 @@ def ensure_tuned(n, precision, kind, ctx=None, batch=1, plan=None):
         return None
     ctx = ctx or _lib.default_context()
     n, precision, kind = int(n), int(precision), int(kind)
-    plan = _lib.default_plan(precision, kind) if plan is None else int(plan)
-    key = (n, precision, kind, int(batch)) if plan == _lib.default_plan(precision, kind) else \
+    plan = _lib.default_plan(precision, kind, n=n) if plan is None else int(plan)
+    key = (n, precision, kind, int(batch)) if plan == _lib.default_plan(precision, kind, n=n) else \
         (n, precision, kind, int(batch), plan)
     with _lock:
         if key in ctx.tunings:

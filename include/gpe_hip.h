@@ -61,11 +61,13 @@ int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime
 
 /* Pass tile variants.  A step is a sequence of passes (kinds 0..GPE_PASS_KINDS-1:
  * z_first, z_mid, z_last, y_fwd, y_inv, x_kin, z_diag, x_diag, z_fwd, y_lz,
- * x_lz, zk_first, zk_mid, yk, xk, xk_last); each pass kind of a solve runs one of the compiled tile variants of
- * its (side, precision), all producing the same transform.  An autotuner
- * times the variants and sets the fastest per kind.  No reference counterpart:
- * the reference's FFT plans are cuFFT's (gpe/make_propagator.py:50,52). */
-enum { GPE_PASS_KINDS = 16 };
+ * x_lz, zk_first, zk_mid, yk, xk, xk_last, zyk_first, zyk_mid); each pass kind
+ * of a solve runs one of the compiled tile variants of its (side, precision),
+ * all producing the same transform (the plane kinds zyk_* have one shape per
+ * side, whatever the variant).  An autotuner times the variants and sets the
+ * fastest per kind.  No reference counterpart: the reference's FFT plans are
+ * cuFFT's (gpe/make_propagator.py:50,52). */
+enum { GPE_PASS_KINDS = 18 };
 /* Bins of the equal-width spectrum view (gpe_spectrum_view). */
 enum { GPE_VIEW_BINS = 128 };
 /* Number of compiled variants for (side, precision); 1 for sides without a
@@ -113,11 +115,16 @@ int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size
  * local half-steps fused into the z pass (and the last one into the x pass):
  * 3 sweeps of psi per step.  GPE_PLAN_FOUR_SWEEP: the 3D FFT, the kinetic
  * factor and the inverse 3D FFT in four passes (z, y, x, y; 5 in imaginary
- * time).  A momentum cutoff (kcut_enable) is not separable, so such solves
- * use, and may only use, the four-sweep plan; every other solve starts on the
- * separable plan.  Batch members share a plan. */
+ * time).  GPE_PLAN_PLANE: the separable plan with the z and y passes of a
+ * step in one, one block per x plane of the field (complex64 at sides 64, 96
+ * and 128, complex128 at 64, where a plane fits the LDS): 2 sweeps of psi per
+ * step, bit-identical to the separable plan.  A momentum cutoff (kcut_enable)
+ * is not separable, so such solves use, and may only use, the four-sweep
+ * plan; every other solve starts on the separable plan.  Batch members share
+ * a plan. */
 #define GPE_PLAN_SEPARABLE 0
 #define GPE_PLAN_FOUR_SWEEP 1
+#define GPE_PLAN_PLANE 2
 int gpe_solve_set_plan(gpe_solve* solve, int32_t plan);
 int gpe_solve_get_plan(gpe_solve* solve, int32_t* plan);
 

@@ -336,24 +336,41 @@ def test_reference_structured_step_is_real_time_only(require_gpu):
     solve.close()
 
 
-def test_stamped_step_matches_the_plain_step_and_reports_every_launch(require_gpu):
+def step_counts(plan, kind, steps):
+    """Launches per pass kind of `steps` steps of one call."""
+    from gpe_simulator_amd import _lib
+    if plan == _lib.PLAN_PLANE:
+        if kind == "real":
+            return {"zyk_first": 1, "zyk_mid": steps - 1, "xk": steps - 1, "xk_last": 1}
+        return {"zyk_first": steps, "xk_last": steps}
+    if kind == "real":
+        return {"zk_first": 1, "zk_mid": steps - 1, "yk": steps, "xk": steps - 1, "xk_last": 1}
+    return {"zk_first": steps, "yk": steps, "xk_last": steps}
+
+
+@pytest.mark.parametrize("plane", [False, True])
+def test_stamped_step_matches_the_plain_step_and_reports_every_launch(plane, require_gpu):
     """gpe_step_stamped advances the state exactly as gpe_step and reports
     one positive span per pass launch (time-dependent tables break the chain
-    of spans and are not charged to any pass)."""
+    of spans and are not charged to any pass), under the separable and the
+    plane plan."""
+    from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import midpoint_times
     n, length, dt, steps = 64, 12.0, 1e-3, 7
     beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)"
     _, stamped = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
     _, plain = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
+    plan = _lib.PLAN_PLANE if plane else _lib.PLAN_SEPARABLE
     psi0 = random_state(n, length, seed=3)
     tm = midpoint_times(0, steps, dt)
     for solve in (stamped, plain):
+        solve.set_plan(plan)
         solve.upload(psi0)
     spans = stamped.step_stamped(steps, tm)
     plain.step(steps, tm)
     np.testing.assert_array_equal(stamped.to_numpy(), plain.to_numpy())
     counts = {k: v[1] for k, v in spans.items() if v[1]}
-    assert counts == {"zk_first": 1, "zk_mid": steps - 1, "yk": steps, "xk": steps - 1, "xk_last": 1}
+    assert counts == step_counts(plan, "real", steps)
     for kind, (us, count) in spans.items():
         if count:
             assert 0 < us / count < 5e3, (kind, us, count)
@@ -361,20 +378,25 @@ def test_stamped_step_matches_the_plain_step_and_reports_every_launch(require_gp
     plain.close()
 
 
+@pytest.mark.parametrize("plane", [False, True])
 @pytest.mark.parametrize("kind", ["real", "imaginary"])
-def test_kernel_timed_step_matches_the_plain_step(kind, require_gpu):
+def test_kernel_timed_step_matches_the_plain_step(kind, plane, require_gpu):
     """gpe_step_kernel_times advances the state exactly as gpe_step and
     reports one positive duration per pass launch, each no longer than the
-    stream interval around the whole call."""
+    stream interval around the whole call, under the separable and the plane
+    plan."""
+    from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import midpoint_times
     n, length, dt, steps = 64, 12.0, 1e-3, 6
     static = "x**2 + y**2 + z**2"
     beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)" if kind == "real" else None
     _, timed = make(n, length, kind, dt, 30.0, static, beam)
     _, plain = make(n, length, kind, dt, 30.0, static, beam)
+    plan = _lib.PLAN_PLANE if plane else _lib.PLAN_SEPARABLE
     psi0 = random_state(n, length, seed=5)
     tm = midpoint_times(0, steps, dt) if beam else None
     for solve in (timed, plain):
+        solve.set_plan(plan)
         solve.upload(psi0)
     timed.mark(0)
     spans = timed.step_kernel_times(steps, tm)
@@ -383,10 +405,7 @@ def test_kernel_timed_step_matches_the_plain_step(kind, require_gpu):
     plain.step(steps, tm)
     np.testing.assert_array_equal(timed.to_numpy(), plain.to_numpy())
     counts = {k: v[1] for k, v in spans.items() if v[1]}
-    if kind == "real":
-        assert counts == {"zk_first": 1, "zk_mid": steps - 1, "yk": steps, "xk": steps - 1, "xk_last": 1}
-    else:
-        assert counts == {"zk_first": steps, "yk": steps, "xk_last": steps}
+    assert counts == step_counts(plan, kind, steps)
     total = 0.0
     for name, (us, count) in spans.items():
         if count:

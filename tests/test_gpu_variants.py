@@ -34,6 +34,17 @@ def variant_count(n, precision):
     return len(_lib.pass_variants(n, _lib.C64 if precision == "c64" else _lib.C128))
 
 
+def plans_for(n, precision):
+    """Every step plan of (side, precision), the default plan last."""
+    from gpe_simulator_amd import _lib
+    prec = _lib.C64 if precision == "c64" else _lib.C128
+    plans = [_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE]
+    if n in _lib.PLANE_SIDES[prec]:
+        plans.append(_lib.PLAN_PLANE)
+    default = _lib.default_plan(prec, _lib.REAL, n=n)
+    return [p for p in plans if p != default] + [default]
+
+
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 @pytest.mark.parametrize("n", SIDES)
 def test_every_variant_matches_the_oracle(n, precision, require_gpu):
@@ -55,7 +66,7 @@ def test_every_variant_matches_the_oracle(n, precision, require_gpu):
         s = make_solve(n, real.x, real.k, "real", dt, beta, static=real.static_source, dynamic=BEAM,
                        precision=precision)
         s.set_variants([v] * len(_lib.PASS_KINDS))
-        for plan in (_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE):     # both step plans, default last
+        for plan in plans_for(n, precision):     # every step plan, default last
             s.set_plan(plan)
             s.upload(psi0)
             s.step(steps, midpoint_times(0, steps, dt))
@@ -71,7 +82,7 @@ def test_every_variant_matches_the_oracle(n, precision, require_gpu):
         s.close()
         s = make_solve(n, imag.x, imag.k, "imaginary", dt, beta, static=imag.static_source, precision=precision)
         s.set_variants([v] * len(_lib.PASS_KINDS))
-        for plan in (_lib.PLAN_FOUR_SWEEP, _lib.PLAN_SEPARABLE):
+        for plan in plans_for(n, precision):
             s.set_plan(plan)
             s.upload(psi0)
             s.step(steps)
