@@ -68,6 +68,7 @@ struct PassArgsRaw {
   int flags;
   int hist_index;
   int stamp_index;
+  int norm_parts;
   double* red;
 };
 constexpr long long kMaxStamps = 1LL << 22;  // wall-clock slots of one stamped call (32 MiB)
@@ -421,6 +422,8 @@ int ensure_red(gpe_solve* s, long long need) {
   return refresh_desc(s);
 }
 
+int norm_kind(int plan);
+
 int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, int hist_index = 0) {
   s->work_is_spectrum = false;            // every pass changes psi or the work field
   int stamp_index = 0;
@@ -439,8 +442,8 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     s->stamp_spans.push_back({kind, begin, begin_count, stamp_index, nblk});
     s->stamp_last = stamp_index;
     s->stamp_last_count = nblk;
-    // the norm finisher follows an imaginary Z_LAST on the stream
-    s->stamp_chained = !(kind == K_Z_LAST && (flags & F_NORM));
+    // the norm finisher follows a norm pass on the stream (unless deferred)
+    s->stamp_chained = !((flags & F_NORM) && !(flags & F_NORM_DEFER));
     flags |= F_STAMP;
   } else if (s->stamping) {
     // out of slots, or a kind outside the step kinds: this launch goes
@@ -449,7 +452,9 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     s->stamp_chained = false;
   }
   flags |= s->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
-  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, s->red_override};
+  const int parts = (flags & F_NORM_FOLD) ? int(s->blocks(norm_kind(s->plan), s->n)) : 0;
+  PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, parts,
+                s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
   if (s->timing && kind < kPassKinds) HIPCHK(s->timer.begin(kind, s->st, &start, &stop));
   else if (s->timing) s->timer.chained = false;   // an untimed kind: the next span starts after it
@@ -641,7 +646,7 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream
 // 1/sqrt(norm) scale to the next load (F_PRESCALE).
 template <class P>
 int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bool prescale_first, bool dyn,
-               P&& pass_fn) {
+               bool fold, P&& pass_fn) {
   const bool sep = plan == GPE_PLAN_SEPARABLE;
   const bool plane = plan == GPE_PLAN_PLANE;
   for (int64_t q = q0; q < q0 + count; ++q) {
@@ -675,20 +680,27 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
         else CHK(pass_fn(K_Z_MID, sa, sb, 0, 0));
       }
     } else {
-      const int pre = (first ? prescale_first : true) ? F_PRESCALE : 0;
+      // With `fold`, every step but the call's last leaves its norm partials
+      // for the next step's first pass to finish (one launch fewer per step);
+      // the last step's finisher publishes norm and scale for the host.
+      const bool folds = fold && q > q0;
+      const bool defers = fold && q + 1 < q0 + count;
+      const int pre = ((first ? prescale_first : true) ? F_PRESCALE : 0) | (folds ? F_NORM_FOLD : 0);
+      const int hpre = folds ? (int)(q - 1) : 0;
+      const int fnorm = F_NORM | (defers ? F_NORM_DEFER : 0);
       if (plane) {
-        CHK(pass_fn(K_ZYK_FIRST, 0, 0, pre, 0));
-        CHK(pass_fn(K_XK_LAST, 0, 0, F_NORM, (int)q));
+        CHK(pass_fn(K_ZYK_FIRST, 0, 0, pre, hpre));
+        CHK(pass_fn(K_XK_LAST, 0, 0, fnorm, (int)q));
       } else if (sep) {
-        CHK(pass_fn(K_ZK_FIRST, 0, 0, pre, 0));
+        CHK(pass_fn(K_ZK_FIRST, 0, 0, pre, hpre));
         CHK(pass_fn(K_YK, 0, 0, 0, 0));
-        CHK(pass_fn(K_XK_LAST, 0, 0, F_NORM, (int)q));
+        CHK(pass_fn(K_XK_LAST, 0, 0, fnorm, (int)q));
       } else {
-        CHK(pass_fn(K_Z_FIRST, 0, 0, pre, 0));
+        CHK(pass_fn(K_Z_FIRST, 0, 0, pre, hpre));
         CHK(pass_fn(K_Y_FWD, 0, 0, 0, 0));
         CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
         CHK(pass_fn(K_Y_INV, 0, 0, 0, 0));
-        CHK(pass_fn(K_Z_LAST, 0, 0, F_NORM, (int)q));
+        CHK(pass_fn(K_Z_LAST, 0, 0, fnorm, (int)q));
       }
     }
   }
@@ -711,17 +723,30 @@ int default_plan(const gpe_solve* s) {
 // The pass kind that writes an imaginary-time step's norm partials.
 int norm_kind(int plan) { return plan == GPE_PLAN_FOUR_SWEEP ? K_Z_LAST : K_XK_LAST; }
 
+// Whether an imaginary step's first pass takes over the norm finisher of the
+// step before it (F_NORM_FOLD): every block of that pass sums all the
+// partials, so only where they are few (small grids, where the finisher's
+// launch is a large share of a step), and the pass's blocks must be whole
+// waves.
+constexpr long long kFoldParts = 1024;
+bool norm_fold_ok(const gpe_solve* s) {
+  if (s->blocks(norm_kind(s->plan), s->n) > kFoldParts) return false;
+  if (s->plan == GPE_PLAN_PLANE) return true;   // a plane block is n pencils of TP threads: whole waves
+  const PassVariant& v = s->variant_of(s->plan == GPE_PLAN_SEPARABLE ? K_ZK_FIRST : K_Z_FIRST);
+  return (v.tp * v.b) % 64 == 0;
+}
+
 int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
   const bool pre = s->pending;
   s->pending = false;
   auto fn = [&](int k, int sa, int sb, int f, int h) { return pass(s, k, false, sa, sb, f, h); };
-  if (!any_dynamic(s)) return emit_steps(GPE_REAL, s->plan, 0, nsteps, nsteps, pre, false, fn);
+  if (!any_dynamic(s)) return emit_steps(GPE_REAL, s->plan, 0, nsteps, nsteps, pre, false, false, fn);
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t cs = std::min<int64_t>(s->chunk, nsteps - done);
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     CHK(eval_dynamic(s, tmid + done, nslots, false));
-    CHK(emit_steps(GPE_REAL, s->plan, done, cs, nsteps, pre, true, fn));
+    CHK(emit_steps(GPE_REAL, s->plan, done, cs, nsteps, pre, true, false, fn));
     done += cs;
   }
   return GPE_OK;
@@ -781,7 +806,7 @@ int step_imag(gpe_solve* s, int64_t nsteps) {
   CHK(ensure_hist(s, nsteps));
   CHK(ensure_red(s, s->blocks(norm_kind(s->plan), s->n)));
   const bool pre = s->pending;
-  CHK(emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
+  CHK(emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   }));
   s->pending = true;
@@ -1378,7 +1403,7 @@ int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
                                      std::to_string(s->hist_cap) + " steps); the allocation-free step cannot grow it");
   const bool pre = s->pending;
   s->red_override = s->ctx->scratch;
-  const int r = emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
+  const int r = emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   });
   s->red_override = nullptr;
@@ -1668,7 +1693,8 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   flags |= b->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
-  PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, nullptr};
+  const int parts = (flags & F_NORM_FOLD) ? int(b->solves[0]->blocks(norm_kind(b->solves[0]->plan), b->n)) : 0;
+  PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, parts, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
   if (b->timing && kind < kPassKinds) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
   else if (b->timing) b->timer.chained = false;   // an untimed kind: the next span starts after it
@@ -1701,7 +1727,8 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   bool pre = false;
   CHK(batch_prepare(b, nsteps, &pre));
   auto emit = [&]() {
-    return emit_steps(b->kind, b->solves[0]->plan, 0, nsteps, nsteps, pre, false, [&](int k, int sa, int sb, int f, int h) {
+    return emit_steps(b->kind, b->solves[0]->plan, 0, nsteps, nsteps, pre, false,
+                      b->kind == GPE_IMAG && norm_fold_ok(b->solves[0]), [&](int k, int sa, int sb, int f, int h) {
       return batch_pass(b, k, sa, sb, f, h);
     });
   };
@@ -1789,7 +1816,7 @@ int gpe_batch_step_times(gpe_batch* b, int64_t nsteps, const double* t_mid, int6
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     for (size_t i = 0; i < b->solves.size(); ++i)
       if (any_dynamic(b->solves[i])) CHK(eval_dynamic(b->solves[i], t_mid + i * stride + done, nslots, false, b->st));
-    CHK(emit_steps(GPE_REAL, b->solves[0]->plan, done, cs, nsteps, pre, true, fn));
+    CHK(emit_steps(GPE_REAL, b->solves[0]->plan, done, cs, nsteps, pre, true, false, fn));
     done += cs;
   }
   batch_finish(b, nsteps);

@@ -133,7 +133,7 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
     case K_Z_FIRST: go<2, K_Z_FIRST>(n, batch, st, a, t0, t1); break;
     case K_Z_MID: go<2, K_Z_MID>(n, batch, st, a, t0, t1); break;
     case K_Z_LAST:
-      if (a.flags & F_NORM) go_norm<2, K_Z_LAST>(n, batch, st, a, t0, t1);
+      if ((a.flags & F_NORM) && !(a.flags & F_NORM_DEFER)) go_norm<2, K_Z_LAST>(n, batch, st, a, t0, t1);
       else go<2, K_Z_LAST>(n, batch, st, a, t0, t1);
       break;
     case K_ZK_FIRST: go<2, K_ZK_FIRST>(n, batch, st, a, t0, t1); break;
@@ -141,7 +141,7 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
     case K_YK: go<1, K_YK>(n, batch, st, a, t0, t1); break;
     case K_XK: go<0, K_XK>(n, batch, st, a, t0, t1); break;
     case K_XK_LAST:
-      if (a.flags & F_NORM) go_norm<0, K_XK_LAST>(n, batch, st, a, t0, t1);
+      if ((a.flags & F_NORM) && !(a.flags & F_NORM_DEFER)) go_norm<0, K_XK_LAST>(n, batch, st, a, t0, t1);
       else go<0, K_XK_LAST>(n, batch, st, a, t0, t1);
       break;
     case K_Z_DIAG: go<2, K_Z_DIAG>(n, batch, st, a, t0, t1); break;

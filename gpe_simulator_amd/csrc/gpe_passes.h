@@ -71,6 +71,10 @@ enum PassFlags : int {
   F_KIN1_LDS = 64, // separable kinds: the launch's dynamic LDS holds the 1D factor table
   F_KIN1_RT = 128, // separable kinds, real time: stage the 1D factor table in LDS where the launcher
                    // finds that it costs no occupancy (the launcher turns it into F_KIN1_LDS)
+  F_NORM_FOLD = 256,  // Z_FIRST / ZK_FIRST / ZYK_FIRST with F_PRESCALE: finish the previous step's norm
+                      // from its partials (norm_parts of them) in every block, instead of k_norm_finish
+  F_NORM_DEFER = 512, // Z_LAST / XK_LAST with F_NORM: write the partials only; the next step's first
+                      // pass finishes the norm (F_NORM_FOLD)
 };
 
 // Per-solve device descriptor.  Lives in device memory; kernels index it with
@@ -116,8 +120,9 @@ struct PassArgs {
   const SolveDev<R>* s;
   int slot_a, slot_b;
   int flags;
-  int hist_index;          // step index into s->hist (Z_LAST with F_NORM)
+  int hist_index;          // step index into s->hist (Z_LAST with F_NORM; the folding pass with F_NORM_FOLD)
   int stamp_index;         // first of this launch's per-block end slots in s->stamps (F_STAMP)
+  int norm_parts;          // F_NORM_FOLD: partials the previous step's last pass wrote per solve
   double* red;             // norm partials of Z_LAST / k_norm_finish; null: the solve's own (s->red)
 };
 
@@ -403,6 +408,44 @@ __device__ __forceinline__ int view_index(int p, int q, int r, int h, int nview,
   return b < nview ? b : nview - 1;
 }
 
+// The norm of the previous imaginary-time step, finished inside the first
+// pass of the next step (F_NORM_FOLD) instead of by k_norm_finish.  Every
+// block sums the FP64 partials exactly as k_norm_finish does (256 strided
+// sums, each wave's shuffle tree, the four wave sums in order from zero), so
+// each block gets the finisher's bits; block 0 publishes norm, scale and
+// history as the finisher would.  The block's threads play the finisher's
+// 256 in turn (blockDim.x a multiple of 64; the launcher folds only then).
+// `sh` is 5 doubles of LDS.  Returns 1/sqrt(norm).
+template <typename R>
+__device__ __forceinline__ double fold_norm(const PassArgs<R>& a, const SolveDev<R>& s, double* sh) {
+  const double* red = a.red ? a.red : s.red;
+  const int nblk = a.norm_parts;
+  const int lane = threadIdx.x & 63;
+  for (int base = 0; base < 256; base += blockDim.x) {
+    const int v = base + int(threadIdx.x);
+    double tot = 0.0;
+    if (v < 256)
+      for (int q = v; q < nblk; q += 256) tot += red[q];
+    tot = wave_sum(tot);
+    if (lane == 0 && v < 256) sh[v >> 6] = tot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int w = 0; w < 4; ++w) sum += sh[w];
+    const double norm = s.dx3 * sum;
+    const double inv = 1.0 / sqrt(norm);
+    sh[4] = inv;
+    if (blockIdx.x == 0) {
+      s.scal[0] = norm;
+      s.scal[1] = inv;
+      if (s.hist) s.hist[a.hist_index] = norm;
+    }
+  }
+  __syncthreads();
+  return sh[4];
+}
+
 // ------------------------------------------------------------ pass kernel --
 // Tile = B pencils; one block per tile.  (A multi-tile variant that prefetched
 // the next tile into registers measured slower at every size: the live
@@ -670,11 +713,12 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   cpx<R>* dst = (a.flags & F_OUT_WORK) ? s.work : s.psi;
   const bool has_vim = s.vim != nullptr;
   const bool prescale = (a.flags & F_PRESCALE) != 0;
+  const bool fold = (a.flags & F_NORM_FOLD) != 0;
   const bool norm = (a.flags & F_NORM) != 0;
   // the pending 1/sqrt(norm) is applied from its FP64 value (float hi + lo
   // for complex64) so the renormalised state has unit norm to rounding, not
   // to the float representation of the scale
-  const rconst<R> sc = make_rconst<R>(prescale ? s.scal[1] : 1.0);
+  rconst<R> sc = make_rconst<R>(prescale && !fold ? s.scal[1] : 1.0);
 
   // Reductions accumulated over all tiles of the block.
   double acc[3] = {0.0, 0.0, 0.0};
@@ -699,6 +743,13 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       for (int m = 0; m < M; ++m) v[m] = gload(tile_elem<TF>(src + g.ubase, off, m, stride));
     } else {
       load_tile<TF>(v, src, g.base, stride, t, n);
+    }
+  }
+  if constexpr (KIND == K_Z_FIRST || KIND == K_ZK_FIRST) {
+    // (while the tile's loads are in flight)
+    if (fold) {
+      __shared__ double fold_lds[5];
+      sc = make_rconst<R>(fold_norm(a, s, fold_lds));
     }
   }
   {

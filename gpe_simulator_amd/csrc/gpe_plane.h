@@ -110,7 +110,11 @@ __global__ __launch_bounds__(TP * N) void k_plane(PassArgs<R> a) {
     for (int m = 0; m < M; ++m) v[m] = gload(src + base + TF::pos(t, m));
     if constexpr (KIND == K_ZYK_FIRST) {
       if (a.flags & F_PRESCALE) {
-        const rconst<R> sc = make_rconst<R>(s.scal[1]);
+        // the previous step's norm from its partials (F_NORM_FOLD), or the
+        // finisher's scale
+        __shared__ double fold_lds[5];
+        const rconst<R> sc =
+            make_rconst<R>((a.flags & F_NORM_FOLD) ? fold_norm(a, s, fold_lds) : s.scal[1]);
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
