@@ -52,7 +52,7 @@ EXPORTS = (
     "gpe_potential_dynamic", "gpe_volume_fill", "gpe_volume_upload", "gpe_step",
     "gpe_norm_history", "gpe_norm", "gpe_normalise", "gpe_diagnostics", "gpe_sync",
     "gpe_step_timed", "gpe_bench_copy", "gpe_angular_momentum", "gpe_batch_create", "gpe_batch_destroy",
-    "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait",
+    "gpe_batch_step", "gpe_batch_step_timed", "gpe_batch_launch", "gpe_batch_wait", "gpe_batch_done",
     "gpe_pass_variants", "gpe_pass_variant_info", "gpe_ctx_set_variants", "gpe_solve_set_variants",
     "gpe_solve_get_variants", "gpe_batch_step_times", "gpe_batch_norm_history", "gpe_device_info",
     "gpe_mark", "gpe_mark_elapsed", "gpe_spectrum_view", "gpe_step_unfused", "gpe_step_stamped",
@@ -139,6 +139,7 @@ def _declare(lib):
         "gpe_batch_step_timed": (c_int, [vp, i64, dp, POINTER(i64)]),
         "gpe_batch_launch": (c_int, [vp, i64, i32]),
         "gpe_batch_wait": (c_int, [vp]),
+        "gpe_batch_done": (c_int, [vp, POINTER(c_int32)]),
         "gpe_pass_variants": (c_int, [i32, i32, POINTER(i32)]),
         "gpe_pass_variant_info": (c_int, [i32, i32, i32, POINTER(i32), POINTER(i32), POINTER(i32)]),
         "gpe_ctx_set_variants": (c_int, [vp, i32, i32, i32, POINTER(i32), i32]),
@@ -573,6 +574,12 @@ class Batch:
 
     def wait(self):
         check(lib().gpe_batch_wait(self.handle))
+
+    def done(self):
+        """True when everything launched on the batch has finished (no wait)."""
+        out = c_int32()
+        check(lib().gpe_batch_done(self.handle, byref(out)))
+        return bool(out.value)
 
     def norm_history(self, count):
         """Per-step norms of the last `count` steps, shape (members, count)."""

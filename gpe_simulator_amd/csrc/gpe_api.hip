@@ -1770,6 +1770,19 @@ int gpe_batch_wait(gpe_batch* b) {
   return GPE_OK;
 }
 
+int gpe_batch_done(gpe_batch* b, int32_t* done) {
+  if (!b || !done) return fail(GPE_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const hipError_t e = hipStreamQuery(b->st);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return GPE_OK;
+  }
+  HIPCHK(e);
+  *done = 1;
+  return GPE_OK;
+}
+
 int gpe_batch_norm_history(gpe_batch* b, double* out, int64_t count) {
   if (!b || (!out && count > 0)) return fail(GPE_ERR_INVALID, "null argument");
   for (gpe_solve* s : b->solves)

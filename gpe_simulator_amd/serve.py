@@ -2,25 +2,30 @@
 
 Requests (imaginary-time ground states to a mu tolerance, or fixed-length
 real-time runs) arrive over time.  The scheduler keeps every admitted field
-resident in HBM and, once per tick:
+resident in HBM and loops:
 
   1. admits waiting requests while free HBM (gpe_mem_info) minus a reserve
      covers the request's resident + diagnostics footprint (gpe_solve_footprint);
-  2. groups the active solves by (grid side, precision, kind, steps this
-     tick) and enqueues each group's steps as one batch (one launch per pass
-     for all members, replayed from a hipGraph), each group on its own HIP
-     stream so groups overlap on the device, then waits for all groups.  A
-     tick is `check_every` steps, or fewer for a real-time run's last tick,
-     which stops on its requested step count;
-  3. reads each member's per-step norms, decides convergence (imaginary time:
-     |dmu| < tol on the norm-decay estimate, with complex64 members moving to
-     a complex128 field when their Switch says so) or completion (real
-     time: the requested step count), computes the FP64 energies of finished
-     solves, returns them and frees their memory.
+  2. groups the idle active solves by (grid side, precision, kind, steps this
+     tick) and enqueues each group's tick as one batch (one launch per pass
+     for all members, replayed from a hipGraph) on its own HIP stream, so
+     groups overlap on the device.  A tick is `check_every` steps, or fewer
+     for a real-time run's last tick, which stops on its requested step
+     count.  At most one group of a shape is in flight: a solve admitted
+     while its shape's group runs joins the group's next tick;
+  3. polls the groups in flight (gpe_batch_done) and, for each one that has
+     finished, reads each member's per-step norms, decides convergence
+     (imaginary time: |dmu| < tol on the norm-decay estimate, with complex64
+     members moving to a complex128 field when their Switch says so) or
+     completion (real time: the requested step count), computes the FP64
+     energies of finished solves, returns them and frees their memory.
 
-A solve's arithmetic does not depend on its batch-mates: every member has its
-own field, tables and reductions, and every tick advances it by the same
-number of steps, so a result is the same alone and in the mix.
+Each group moves at its own pace: a 64^3 group's tick does not wait for a
+256^3 group's, so a small request's latency follows its own work, not the
+largest solve on the device.  A solve's arithmetic does not depend on its
+batch-mates: every member has its own field, tables and reductions, and
+every tick advances it by the same number of steps, so a result is the same
+alone and in the mix.
 """
 
 from __future__ import annotations
@@ -80,6 +85,7 @@ class _Slot:
     steps: int = 0
     dmu: float = float("inf")
     switch: Switch = None
+    busy: bool = False         # a member of a group in flight
 
 
 def _footprint(req, precision):
@@ -137,89 +143,104 @@ class Server:
             self._batches.pop(key).close()
 
     # -- main loop
-    def run(self, requests, realtime=True, log=None):
-        """Serve `requests` (arrival-ordered) and return (outcomes, stats)."""
+    def run(self, requests, realtime=True, log=None, poll_s=1e-4):
+        """Serve `requests` (arrival-ordered) and return (outcomes, stats).
+        realtime=False runs on a virtual clock: arrivals are not waited for,
+        and the clock advances by the host time the loop spends."""
         for req in requests:
             if req.kind == "real" and req.steps < 1:
                 raise ValueError(f"request {req.id}: a real-time run needs at least one step")
         start = time.perf_counter()
         incoming = deque(sorted(requests, key=lambda r: r.arrival))
         waiting, active, outcomes = deque(), [], []
-        clock = (lambda: time.perf_counter() - start) if realtime else None
+        inflight = []              # (shape, steps, members, batch)
         virtual = 0.0
+        last = time.perf_counter()
+
+        def now():
+            nonlocal virtual, last
+            if realtime:
+                return time.perf_counter() - start
+            t = time.perf_counter()
+            virtual += t - last
+            last = t
+            return virtual
+
         while incoming or waiting or active:
-            now = clock() if realtime else virtual
-            while incoming and incoming[0].arrival <= now:
+            t = now()
+            while incoming and incoming[0].arrival <= t:
                 waiting.append(incoming.popleft())
             if not active and not waiting:
                 if realtime:
-                    time.sleep(max(0.0, incoming[0].arrival - now))
+                    time.sleep(max(0.0, incoming[0].arrival - t))
                 else:
-                    virtual = incoming[0].arrival
+                    virtual = max(virtual, incoming[0].arrival)
                 continue
             # admission by free device memory
-            free, _total = self.ctx.mem_info()
-            for req in list(waiting):
-                need = _footprint(req, req.precision)
-                if req.precision == "c64" and req.kind == "imaginary":
-                    need += _footprint(req, "c128")   # room for the complex128 polish
-                if free - self.reserve >= need or not active:
-                    waiting.remove(req)
-                    active.append(self._admit(req, now))
-                    free -= need
-            self.max_active = max(self.max_active, len(active))
-            # one tick: every group enqueued, then all awaited.  A real-time
-            # member steps at most its remaining count, so a run of any length
-            # ends on its requested step; members with equal step counts share
-            # a group.
+            if waiting:
+                free, _total = self.ctx.mem_info()
+                for req in list(waiting):
+                    need = _footprint(req, req.precision)
+                    if req.precision == "c64" and req.kind == "imaginary":
+                        need += _footprint(req, "c128")   # room for the complex128 polish
+                    if free - self.reserve >= need or not active:
+                        waiting.remove(req)
+                        active.append(self._admit(req, t))
+                        free -= need
+                self.max_active = max(self.max_active, len(active))
+            # launch the idle members of every shape without a group in flight;
+            # a real-time member steps at most its remaining count, so a run of
+            # any length ends on its requested step, and members with equal
+            # step counts share a group
+            running = {g[0] for g in inflight}
             groups = {}
             for slot in active:
-                steps = self._tick_steps(slot)
-                groups.setdefault((slot.req.n, slot.precision, slot.req.kind, steps), []).append(slot)
-            tick0 = time.perf_counter()
+                shape = (slot.req.n, slot.precision, slot.req.kind)
+                if not slot.busy and shape not in running:
+                    groups.setdefault(shape + (self._tick_steps(slot),), []).append(slot)
             # every batch is built (device allocation) before any group runs
-            launched = [(key[3], members, self._batch_for(members)) for key, members in groups.items()]
-            for steps, _, batch in launched:
+            ready = [(key[:3], key[3], members, self._batch_for(members)) for key, members in groups.items()]
+            for shape, steps, members, batch in ready:
                 batch.launch(steps, graph=self.graph)
-            for _, _, batch in launched:
-                batch.wait()
-            # per-step norms of every imaginary-time member, one fetch per group
-            norms = {}
-            for steps, members, batch in launched:
-                if members[0].req.kind == "imaginary":
-                    for slot, row in zip(members, batch.norm_history(steps)):
-                        norms[id(slot)] = row
-            stepped = {id(slot): steps for steps, members, _ in launched for slot in members}
-            if not realtime:
-                # without sleeping for arrivals, time still passes while the device works
-                virtual += time.perf_counter() - tick0
-            # progress, convergence and completion
-            finished = []
-            for slot in active:
-                slot.steps += stepped[id(slot)]
-                self.grid_point_steps += slot.req.n ** 3 * stepped[id(slot)]
-                req = slot.req
-                if req.kind == "imaginary":
-                    try:
-                        mu = mu_from_norms(norms[id(slot)], req.dt)
-                    except FloatingPointError as err:
-                        raise FloatingPointError(f"request {req.id} (n={req.n}, {slot.precision}, "
-                                                 f"{slot.steps} steps): {err}") from None
-                    slot.dmu = float(abs(mu[-1] - mu[-2]))
-                    if slot.steps >= req.max_steps:
+                for slot in members:
+                    slot.busy = True
+                inflight.append((shape, steps, members, batch))
+            # poll: handle every group that has finished; sleep briefly when none has
+            done = [g for g in inflight if g[3].done()]
+            if not done:
+                time.sleep(poll_s)
+                continue
+            for group in done:
+                inflight.remove(group)
+                _shape, steps, members, batch = group
+                norms = batch.norm_history(steps) if members[0].req.kind == "imaginary" else None
+                finished = []
+                for j, slot in enumerate(members):
+                    slot.busy = False
+                    slot.steps += steps
+                    self.grid_point_steps += slot.req.n ** 3 * steps
+                    req = slot.req
+                    if req.kind == "imaginary":
+                        try:
+                            mu = mu_from_norms(norms[j], req.dt)
+                        except FloatingPointError as err:
+                            raise FloatingPointError(f"request {req.id} (n={req.n}, {slot.precision}, "
+                                                     f"{slot.steps} steps): {err}") from None
+                        slot.dmu = float(abs(mu[-1] - mu[-2]))
+                        if slot.steps >= req.max_steps:
+                            finished.append(slot)
+                        elif slot.precision == "c64":
+                            if slot.switch.update(slot.steps, slot.dmu):
+                                self._promote(slot)
+                        elif slot.dmu < req.tol:
+                            finished.append(slot)
+                    elif slot.steps >= req.steps:
                         finished.append(slot)
-                    elif slot.precision == "c64":
-                        if slot.switch.update(slot.steps, slot.dmu):
-                            self._promote(slot)
-                    elif slot.dmu < req.tol:
-                        finished.append(slot)
-                elif slot.steps >= req.steps:
-                    finished.append(slot)
-            for slot in finished:
-                outcomes.append(self._complete(slot, clock() if realtime else virtual))
-                active.remove(slot)
-                if log:
-                    log(outcomes[-1])
+                for slot in finished:
+                    outcomes.append(self._complete(slot, now()))
+                    active.remove(slot)
+                    if log:
+                        log(outcomes[-1])
         elapsed = time.perf_counter() - start
         lat = np.array([o.latency for o in outcomes]) * 1e3
         stats = dict(requests=len(outcomes), seconds=elapsed,

@@ -276,6 +276,10 @@ int gpe_batch_step_times(gpe_batch* batch, int64_t nsteps, const double* t_mid, 
  * until the wait returns. */
 int gpe_batch_launch(gpe_batch* batch, int64_t nsteps, int32_t use_graph);
 int gpe_batch_wait(gpe_batch* batch);
+/* Non-blocking form of the wait: *done = 1 when everything the batch has
+ * enqueued has finished, else 0.  A server polls its batches with it and
+ * relaunches each one as soon as its own steps are done. */
+int gpe_batch_done(gpe_batch* batch, int32_t* done);
 /* Per-step norms of the last `count` steps of every member (imaginary time),
  * member-major: out[i*count + j]; one synchronisation for the whole batch.
  * Reference: the per-step normalise of make_imag_propagator
