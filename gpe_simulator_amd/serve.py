@@ -11,7 +11,11 @@ resident in HBM and loops:
      for all members, replayed from a hipGraph) on its own HIP stream, so
      groups overlap on the device.  A tick is `check_every` steps, or fewer
      for a real-time run's last tick, which stops on its requested step
-     count.  At most one group of a shape is in flight: a solve admitted
+     count.  A large shape's tick goes to the device in pieces of about
+     `launch_budget_s` each, each piece launched when the one before has
+     finished, so that the launches of small groups, which share the
+     process's few hardware queues, are not queued behind a whole large
+     tick.  At most one group of a shape is in flight: a solve admitted
      while its shape's group runs joins the group's next tick;
   3. polls the groups in flight (gpe_batch_done) and, for each one that has
      finished, reads each member's per-step norms, decides convergence
@@ -95,9 +99,14 @@ def _footprint(req, precision):
 
 
 class Server:
-    def __init__(self, ctx=None, check_every=50, reserve_bytes=2 << 30, graph=True):
+    # seconds per grid point and step of a complex64 step (256^3 runs at about
+    # 1.1e11 grid-point-steps/s), for sizing a tick's launches; complex128 twice
+    STEP_SECONDS_PER_POINT = 9e-12
+
+    def __init__(self, ctx=None, check_every=50, reserve_bytes=2 << 30, graph=True, launch_budget_s=2.5e-3):
         self.ctx = ctx or _lib.default_context()
         self.check_every = int(check_every)
+        self.launch_budget_s = launch_budget_s
         self.reserve = reserve_bytes
         self.graph = graph
         self._batches = {}
@@ -143,7 +152,7 @@ class Server:
             self._batches.pop(key).close()
 
     # -- main loop
-    def run(self, requests, realtime=True, log=None, poll_s=1e-4):
+    def run(self, requests, realtime=True, log=None, poll_s=2e-5):
         """Serve `requests` (arrival-ordered) and return (outcomes, stats).
         realtime=False runs on a virtual clock: arrivals are not waited for,
         and the clock advances by the host time the loop spends."""
@@ -153,7 +162,7 @@ class Server:
         start = time.perf_counter()
         incoming = deque(sorted(requests, key=lambda r: r.arrival))
         waiting, active, outcomes = deque(), [], []
-        inflight = []              # (shape, steps, members, batch)
+        inflight = []              # [shape, tick steps, steps launched, last launch, members, batch]
         virtual = 0.0
         last = time.perf_counter()
 
@@ -201,19 +210,32 @@ class Server:
             # every batch is built (device allocation) before any group runs
             ready = [(key[:3], key[3], members, self._batch_for(members)) for key, members in groups.items()]
             for shape, steps, members, batch in ready:
-                batch.launch(steps, graph=self.graph)
+                first = self._launch_steps(shape, steps)
+                batch.launch(first, graph=self.graph)
                 for slot in members:
                     slot.busy = True
-                inflight.append((shape, steps, members, batch))
-            # poll: handle every group that has finished; sleep briefly when none has
-            done = [g for g in inflight if g[3].done()]
+                inflight.append([shape, steps, first, first, members, batch])
+            # poll: launch the next piece of every group whose piece has finished
+            # and handle every finished tick; sleep briefly when nothing has
+            done = []
+            for group in inflight:
+                if not group[5].done():
+                    continue
+                shape, steps, launched = group[0], group[1], group[2]
+                if launched < steps:
+                    piece = min(self._launch_steps(shape, steps), steps - launched)
+                    group[5].launch(piece, graph=self.graph)
+                    group[2], group[3] = launched + piece, piece
+                else:
+                    done.append(group)
             if not done:
                 time.sleep(poll_s)
                 continue
             for group in done:
                 inflight.remove(group)
-                _shape, steps, members, batch = group
-                norms = batch.norm_history(steps) if members[0].req.kind == "imaginary" else None
+                _shape, steps, _launched, last, members, batch = group
+                # the last piece's norms: the estimate needs the tick's last two steps
+                norms = batch.norm_history(last) if members[0].req.kind == "imaginary" else None
                 finished = []
                 for j, slot in enumerate(members):
                     slot.busy = False
@@ -252,6 +274,21 @@ class Server:
             batch.close()
         self._batches.clear()
         return outcomes, stats
+
+    def _launch_steps(self, shape, tick):
+        """Steps of one launch of a `tick`-step tick of `shape` (side,
+        precision, kind): the whole tick when it fits the launch budget, else
+        the largest divisor of the tick that does (so the pieces are equal and
+        one captured graph serves them), at least 2 (the mu estimate reads the
+        last piece's last two norms).  A function of the shape alone, so a
+        member's launches are the same alone and in the mix."""
+        n, precision, _kind = shape
+        step_s = n ** 3 * self.STEP_SECONDS_PER_POINT * (2.0 if precision == "c128" else 1.0)
+        cap = max(2, int(self.launch_budget_s / step_s)) if self.launch_budget_s else tick
+        if tick <= cap:
+            return tick
+        best = max(d for d in range(1, cap + 1) if tick % d == 0)
+        return best if best >= max(2, cap // 2) else cap
 
     def _tick_steps(self, slot):
         """Steps `slot` takes this tick: check_every, or a real-time run's

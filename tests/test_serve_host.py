@@ -108,6 +108,16 @@ def imag(i, n, arrival=0.0):
                    tol=1e-6)
 
 
+def test_large_real_run_ends_on_its_step_count_in_pieces(fake_device):
+    """A 256^3 complex64 real-time run of 133 steps: ticks of 50, 50 and 33
+    steps, each sent as launches within the budget."""
+    req = Request(id=0, arrival=0.0, n=256, kind="real", precision="c64", beta=100.0, steps=133)
+    outcomes, _ = Server(ctx=FakeContext(), check_every=50).run([req])
+    assert outcomes[0].steps == 133 and fake_device[0].steps == 133
+    pieces = [steps for _, steps, _ in FakeBatch.launches]
+    assert sum(pieces) == 133 and max(pieces) <= 16
+
+
 def test_real_run_ends_on_its_step_count(fake_device):
     req = Request(id=0, arrival=0.0, n=64, kind="real", precision="c64", beta=100.0, steps=120)
     outcomes, _ = Server(ctx=FakeContext(), check_every=50).run([req])
@@ -130,6 +140,12 @@ def test_small_requests_do_not_wait_for_a_large_one(fake_device):
     assert lat[0] >= (steps[0] // 50) * big_tick * 0.9
     # equal-shape members share one group; never two groups of a shape at once
     assert max(FakeBatch.most_in_flight.values()) == 1
+    # the large shape's 50-step ticks go to the device in equal pieces that fit
+    # the launch budget (5 steps of a 256^3 complex128 solve), the small
+    # shapes' in one launch
+    assert {steps for shape, steps, _ in FakeBatch.launches if shape[0] == 256} == {5}
+    assert {steps for shape, steps, _ in FakeBatch.launches if shape[0] == 64} == {50}
+    assert steps[0] % 50 == 0
     assert any(shape[0] == 64 and members == 3 for shape, _, members in FakeBatch.launches)
     assert all(o.converged for o in outcomes)
     assert all(s.closed for s in fake_device)
