@@ -15,8 +15,9 @@ resident in HBM and loops:
      `launch_budget_s` each, each piece launched when the one before has
      finished, so that the launches of small groups, which share the
      process's few hardware queues, are not queued behind a whole large
-     tick.  At most one group of a shape is in flight: a solve admitted
-     while its shape's group runs joins the group's next tick;
+     tick (while other groups are in flight; a group alone sends the rest
+     of its tick at once).  At most one group of a shape is in flight: a
+     solve admitted while its shape's group runs joins the group's next tick;
   3. polls the groups in flight (gpe_batch_done) and, for each one that has
      finished, reads each member's per-step norms, decides convergence
      (imaginary time: |dmu| < tol on the norm-decay estimate, with complex64
@@ -221,11 +222,17 @@ class Server:
             for group in inflight:
                 if not group[5].done():
                     continue
-                shape, steps, launched = group[0], group[1], group[2]
-                if launched < steps:
-                    piece = min(self._launch_steps(shape, steps), steps - launched)
-                    group[5].launch(piece, graph=self.graph)
-                    group[2], group[3] = launched + piece, piece
+                shape, steps = group[0], group[1]
+                if group[2] < steps:
+                    # pieces one at a time while other groups share the device;
+                    # alone, the rest of the tick at once (no gaps)
+                    contended = len(inflight) > 1 or bool(waiting)
+                    while group[2] < steps:
+                        piece = min(self._launch_steps(shape, steps), steps - group[2])
+                        group[5].launch(piece, graph=self.graph)
+                        group[2], group[3] = group[2] + piece, piece
+                        if contended:
+                            break
                 else:
                     done.append(group)
             if not done:

@@ -54,11 +54,16 @@ class FakeBatch:
         self.end = None
 
     def launch(self, steps, graph=True):
-        assert self.end is None, "a batch is launched again before its tick finished"
+        # launches queue on the batch's stream: a second one starts when the first ends
         for s in self.solves:
             assert not s.closed
             s.steps += steps
-        self.end = time.perf_counter() + steps * SECONDS_PER_STEP[self.shape[0]]
+        queued = self.end is not None
+        start = self.end if queued else time.perf_counter()
+        self.end = start + steps * SECONDS_PER_STEP[self.shape[0]]
+        if queued:
+            FakeBatch.launches.append((self.shape, steps, len(self.solves)))
+            return
         self.steps = steps
         running = FakeBatch.in_flight.get(self.shape, 0) + 1
         FakeBatch.in_flight[self.shape] = running
@@ -110,7 +115,8 @@ def imag(i, n, arrival=0.0):
 
 def test_large_real_run_ends_on_its_step_count_in_pieces(fake_device):
     """A 256^3 complex64 real-time run of 133 steps: ticks of 50, 50 and 33
-    steps, each sent as launches within the budget."""
+    steps, each sent as launches within the budget (alone, each tick's pieces
+    are queued at once)."""
     req = Request(id=0, arrival=0.0, n=256, kind="real", precision="c64", beta=100.0, steps=133)
     outcomes, _ = Server(ctx=FakeContext(), check_every=50).run([req])
     assert outcomes[0].steps == 133 and fake_device[0].steps == 133
