@@ -32,7 +32,7 @@ PLAN_SEPARABLE, PLAN_FOUR_SWEEP, PLAN_PLANE = 0, 1, 2
 
 
 # sides with a plane pass (an x plane of the field fits the LDS) per precision
-PLANE_SIDES = {0: (64, 96, 128), 1: (64,)}
+PLANE_SIDES = {0: (32, 48, 64, 96, 128), 1: (32, 48, 64)}
 
 
 def default_plan(precision, kind, masked=False, n=None):

@@ -183,6 +183,8 @@ struct FastTF {
     return B * pencil_stride<AXIS>();
   }
   static_assert(N % TP == 0 && M % TP == 0, "N must be TP*TP*Q");
+  // X/Y tiles are B consecutive pencils of one row of n
+  static_assert(N % B == 0, "B pencils per block must divide N");
 
   __device__ static __forceinline__ int pos(int t, int m) { return t + TP * m; }
   __device__ static __forceinline__ bool valid(int, int, int) { return true; }

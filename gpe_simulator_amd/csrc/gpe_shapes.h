@@ -21,6 +21,8 @@
 #include "gpe_passes.h"
 
 #define GPE_VARIANTS(X)                                                                                   \
+  X(0, 32, 0, 4, 16, 4, 1) X(0, 32, 1, 4, 32, 4, 1)                                                       \
+  X(0, 48, 0, 4, 16, 4, 1) X(0, 48, 1, 4, 48, 4, 1)                                                       \
   X(0, 64, 0, 8, 32, 3, 1) X(0, 64, 1, 8, 16, 4, 1) X(0, 64, 2, 8, 32, 4, 1)                              \
   X(0, 96, 0, 4, 32, 3, 1) X(0, 96, 1, 4, 16, 3, 1) X(0, 96, 2, 4, 32, 4, 1) X(0, 96, 3, 4, 32, 3, 2)     \
   X(0, 128, 0, 8, 32, 3, 1) X(0, 128, 1, 8, 16, 3, 1) X(0, 128, 2, 8, 16, 4, 1)                           \
@@ -28,6 +30,8 @@
   X(0, 256, 0, 16, 16, 3, 1) X(0, 256, 1, 16, 16, 4, 1) X(0, 256, 2, 16, 32, 3, 1)                        \
   X(0, 512, 0, 16, 8, 2, 1) X(0, 512, 1, 16, 16, 2, 1)                                                     \
   X(0, 1024, 0, 32, 4, 1, 1) X(0, 1024, 1, 32, 8, 1, 1)                                                    \
+  X(1, 32, 0, 4, 16, 2, 1) X(1, 32, 1, 4, 32, 2, 1)                                                       \
+  X(1, 48, 0, 4, 16, 2, 1) X(1, 48, 1, 4, 48, 2, 1)                                                       \
   X(1, 64, 0, 8, 16, 2, 1) X(1, 64, 1, 8, 32, 2, 1)                                                       \
   X(1, 96, 0, 4, 16, 2, 1) X(1, 96, 1, 4, 32, 2, 1) X(1, 96, 2, 4, 8, 2, 1) X(1, 96, 3, 4, 32, 2, 2)      \
   X(1, 128, 0, 8, 16, 2, 1) X(1, 128, 1, 8, 32, 2, 1) X(1, 128, 2, 8, 8, 2, 1)                            \

@@ -15,7 +15,7 @@ from oracle import gpe_oracle as orc
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
 BEAM = "10*exp(-((x-4*cos(1.0*t))**2+(y-4*sin(1.0*t))**2)/2)"
-SIDES = [64, 96, 128, 192, 256]
+SIDES = [32, 48, 64, 96, 128, 192, 256]
 
 
 @pytest.fixture(autouse=True, scope="module")
