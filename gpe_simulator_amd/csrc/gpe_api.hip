@@ -11,6 +11,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "gpe_hip.h"
@@ -317,15 +318,75 @@ int set_device(gpe_solve* s) {
   return GPE_OK;
 }
 
+// ------------------------------------------------------ device block pool --
+// Blocks that destroyed solves and batches held, kept per device for the next
+// allocation of the same size.  A solve's buffers have sizes set by its shape,
+// so sweeps and servers that create and destroy solves of a few shapes reuse
+// them instead of paying one hipFree (an unmap) per buffer and one hipMalloc
+// per new buffer.  Blocks enter the pool only once the device has finished
+// all work on them (destroy synchronises first), leave it on an allocation of
+// the same size (dzalloc then zero-fills them on the new owner's stream), and
+// are released when an allocation fails, when the pool would pass its cap,
+// and when a context of the device is destroyed.  gpe_mem_info counts pooled
+// bytes as free.
+struct BlockPool {
+  std::multimap<size_t, void*> blocks;
+  size_t bytes = 0;
+};
+std::mutex g_pool_lock;
+BlockPool g_pool[kMaxDevices];
+std::unordered_map<void*, size_t> g_block_size;   // bytes of every live dmalloc block
+constexpr size_t kPoolCap = size_t(32) << 30;      // bytes one device's pool may hold
+
+int current_device() {
+  int d = -1;
+  return hipGetDevice(&d) == hipSuccess && d >= 0 && d < kMaxDevices ? d : -1;
+}
+
+// (g_pool_lock held)
+void release_pool(int dev) {
+  for (auto& kv : g_pool[dev].blocks) (void)hipFree(kv.second);
+  g_pool[dev].blocks.clear();
+  g_pool[dev].bytes = 0;
+}
+
+size_t pooled_bytes(int dev) {
+  std::lock_guard<std::mutex> guard(g_pool_lock);
+  return dev >= 0 && dev < kMaxDevices ? g_pool[dev].bytes : 0;
+}
+
 int dmalloc(void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  const int dev = current_device();
+  if (dev >= 0) {
+    std::lock_guard<std::mutex> guard(g_pool_lock);
+    auto it = g_pool[dev].blocks.find(bytes);
+    if (it != g_pool[dev].blocks.end()) {
+      *p = it->second;
+      g_pool[dev].blocks.erase(it);
+      g_pool[dev].bytes -= bytes;
+      g_block_size[*p] = bytes;
+      return GPE_OK;
+    }
+  }
   hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && dev >= 0) {
+    // the pooled blocks may be what is missing
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> guard(g_pool_lock);
+      release_pool(dev);
+    }
+    e = hipMalloc(p, bytes);
+  }
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return fail(GPE_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
                                    hipGetErrorString(e));
   }
+  std::lock_guard<std::mutex> guard(g_pool_lock);
+  g_block_size[*p] = bytes;
   return GPE_OK;
 }
 
@@ -342,7 +403,28 @@ int dzalloc(T** p, size_t bytes, hipStream_t st) {
 }
 
 void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> guard(g_pool_lock);
+    g_block_size.erase(p);
+  }
+  (void)hipFree(p);
+}
+
+// Returns a dmalloc block to device `dev`'s pool.  Only once no work on the
+// device can still touch it.
+void drecycle(int dev, void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> guard(g_pool_lock);
+  auto it = g_block_size.find(p);
+  const size_t bytes = it == g_block_size.end() ? 0 : it->second;
+  if (it != g_block_size.end()) g_block_size.erase(it);
+  if (bytes == 0 || dev < 0 || dev >= kMaxDevices || g_pool[dev].bytes + bytes > kPoolCap) {
+    (void)hipFree(p);
+    return;
+  }
+  g_pool[dev].blocks.emplace(bytes, p);
+  g_pool[dev].bytes += bytes;
 }
 
 template <typename R>
@@ -956,6 +1038,12 @@ int gpe_ctx_create(int device, gpe_ctx** out) {
 }
 
 int gpe_ctx_destroy(gpe_ctx* ctx) {
+  if (!ctx) return GPE_OK;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    (void)hipSetDevice(ctx->device);
+    std::lock_guard<std::mutex> guard(g_pool_lock);
+    release_pool(ctx->device);
+  }
   delete ctx;
   return GPE_OK;
 }
@@ -964,6 +1052,7 @@ int gpe_mem_info(gpe_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
   if (!ctx || !free_bytes || !total_bytes) return fail(GPE_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  *free_bytes += pooled_bytes(ctx->device);   // pooled blocks serve the next allocations
   return GPE_OK;
 }
 
@@ -1169,15 +1258,19 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
 int gpe_solve_destroy(gpe_solve* s) {
   if (!s) return GPE_OK;
   (void)hipSetDevice(s->ctx->device);
-  if (s->st) (void)hipStreamSynchronize(s->st);
+  // the buffers go to the device's block pool, so nothing on the device may
+  // still use them: the solve's stream, or a batch's that holds the solve
+  // (one device-wide wait, which each hipFree would otherwise make)
+  (void)hipDeviceSynchronize();
+  const int dev = s->ctx->device;
   void* ptrs[] = {s->slab, s->work, s->m_vol[0], s->m_vol[1], s->w_vol[0], s->w_vol[1], s->d_tab[0], s->d_tab[1],
                   s->d_tab[2], s->g_tab[0], s->g_tab[1], s->g_tab[2], s->d_tlist, s->red, s->hist, s->nkocc,
                   s->nkcnt, s->d_out, s->d_nkocc, s->d_nkcnt, s->stage, s->stamps, s->mon};
-  for (void* p : ptrs) dfree(p);
+  for (void* p : ptrs) drecycle(dev, p);
   for (auto& p : s->dyn) {
-    dfree(p.code);
-    dfree(p.consts);
-    dfree(p.hv);
+    drecycle(dev, p.code);
+    drecycle(dev, p.consts);
+    drecycle(dev, p.hv);
   }
   for (int f = 0; f < 2; ++f) {
     if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
@@ -1616,7 +1709,7 @@ int gpe_batch_destroy(gpe_batch* b) {
   if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  dfree(b->desc);
+  drecycle(b->ctx->device, b->desc);   // (the batch's stream, the only user, has finished)
   b->timer.destroy();
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
