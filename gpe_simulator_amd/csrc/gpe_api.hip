@@ -218,7 +218,68 @@ struct gpe_ctx {
   double* scratch = nullptr;              // this device's `scratch` (device address)
   // per-pass-kind variant defaults for new solves, keyed by (side, precision, kind)
   std::map<std::tuple<int, int, int>, std::array<int, kPassKinds>> variants;
+  // streams and pinned time-list buffers of destroyed solves and batches,
+  // handed to the next ones (creating and destroying them costs about as
+  // much as the rest of a solve's set-up)
+  std::mutex spare_lock;
+  std::vector<hipStream_t> spare_streams;
+  std::multimap<size_t, void*> spare_pinned;
 };
+
+namespace {
+constexpr size_t kSpareStreams = 256, kSparePinned = 512;
+
+hipError_t take_stream(gpe_ctx* ctx, hipStream_t* st) {
+  {
+    std::lock_guard<std::mutex> guard(ctx->spare_lock);
+    if (!ctx->spare_streams.empty()) {
+      *st = ctx->spare_streams.back();
+      ctx->spare_streams.pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+// (the stream has finished its work)
+void give_stream(gpe_ctx* ctx, hipStream_t st) {
+  if (!st) return;
+  {
+    std::lock_guard<std::mutex> guard(ctx->spare_lock);
+    if (ctx->spare_streams.size() < kSpareStreams) {
+      ctx->spare_streams.push_back(st);
+      return;
+    }
+  }
+  (void)hipStreamDestroy(st);
+}
+
+hipError_t take_pinned(gpe_ctx* ctx, size_t bytes, void** p) {
+  {
+    std::lock_guard<std::mutex> guard(ctx->spare_lock);
+    auto it = ctx->spare_pinned.find(bytes);
+    if (it != ctx->spare_pinned.end()) {
+      *p = it->second;
+      ctx->spare_pinned.erase(it);
+      return hipSuccess;
+    }
+  }
+  return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+
+// (no copy from the buffer is in flight)
+void give_pinned(gpe_ctx* ctx, size_t bytes, void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> guard(ctx->spare_lock);
+    if (ctx->spare_pinned.size() < kSparePinned) {
+      ctx->spare_pinned.emplace(bytes, p);
+      return;
+    }
+  }
+  (void)hipHostFree(p);
+}
+}  // namespace
 
 // Pass kinds outside the tuned set (the reference-structured step) run the
 // variant chosen for the tuned kind along the same axis.
@@ -249,6 +310,7 @@ struct gpe_solve {
   }
 
   void* slab = nullptr;                   // psi, tables and descriptors (one allocation)
+  size_t h_tlist_bytes = 0;               // bytes of each pinned time-list buffer
   void* psi = nullptr;
   void* work = nullptr;
   void* tw = nullptr;
@@ -660,11 +722,13 @@ int layout_dynamic(gpe_solve* s) {
   s->d_tlist = nullptr;
   CHK(dzalloc(&s->d_tlist, (size_t)(chunk + 2) * sizeof(double), s->st));
   for (int f = 0; f < 2; ++f) {
-    if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
+    // (the stream has finished: no copy from the old buffers is in flight)
+    give_pinned(s->ctx, s->h_tlist_bytes, s->h_tlist[f]);
     s->h_tlist[f] = nullptr;
-    HIPCHK(hipHostMalloc((void**)&s->h_tlist[f], (size_t)(chunk + 2) * sizeof(double), hipHostMallocDefault));
+    HIPCHK(take_pinned(s->ctx, (size_t)(chunk + 2) * sizeof(double), (void**)&s->h_tlist[f]));
     if (!s->tlist_ev[f]) HIPCHK(hipEventCreateWithFlags(&s->tlist_ev[f], hipEventDisableTiming));
   }
+  s->h_tlist_bytes = (size_t)(chunk + 2) * sizeof(double);
   return refresh_desc(s);
 }
 
@@ -1044,6 +1108,8 @@ int gpe_ctx_destroy(gpe_ctx* ctx) {
     std::lock_guard<std::mutex> guard(g_pool_lock);
     release_pool(ctx->device);
   }
+  for (hipStream_t st : ctx->spare_streams) (void)hipStreamDestroy(st);
+  for (auto& kv : ctx->spare_pinned) (void)hipHostFree(kv.second);
   delete ctx;
   return GPE_OK;
 }
@@ -1206,7 +1272,7 @@ int gpe_solve_create(gpe_ctx* ctx, const gpe_solve_desc* d, gpe_solve** out) {
     gpe_solve_destroy(s);
     return code;
   };
-  hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
+  hipError_t e = take_stream(ctx, &s->st);
   if (e != hipSuccess) {
     delete s;
     return fail(GPE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -1273,13 +1339,13 @@ int gpe_solve_destroy(gpe_solve* s) {
     drecycle(dev, p.hv);
   }
   for (int f = 0; f < 2; ++f) {
-    if (s->h_tlist[f]) (void)hipHostFree(s->h_tlist[f]);
+    give_pinned(s->ctx, s->h_tlist_bytes, s->h_tlist[f]);
     if (s->tlist_ev[f]) (void)hipEventDestroy(s->tlist_ev[f]);
   }
   s->timer.destroy();
   for (hipEvent_t e : s->mark)
     if (e) (void)hipEventDestroy(e);
-  if (s->st) (void)hipStreamDestroy(s->st);
+  give_stream(s->ctx, s->st);
   delete s;
   return GPE_OK;
 }
@@ -1711,7 +1777,7 @@ int gpe_batch_destroy(gpe_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   drecycle(b->ctx->device, b->desc);   // (the batch's stream, the only user, has finished)
   b->timer.destroy();
-  if (b->st) (void)hipStreamDestroy(b->st);
+  give_stream(b->ctx, b->st);
   delete b;
   return GPE_OK;
 }
@@ -1737,8 +1803,7 @@ int gpe_batch_create(gpe_ctx* ctx, gpe_solve* const* solves, int32_t count, gpe_
     gpe_batch_destroy(b);
     return code;
   };
-  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(GPE_ERR_HIP, "batch stream creation failed"));
+  if (take_stream(ctx, &b->st) != hipSuccess) return bail(fail(GPE_ERR_HIP, "batch stream creation failed"));
   const size_t dsz = b->prec == GPE_C64 ? sizeof(SolveDev<float>) : sizeof(SolveDev<double>);
   if (int r = dmalloc(&b->desc, dsz * count)) return bail(r);
   *out = b;
