@@ -1933,6 +1933,8 @@ int gpe_batch_done(gpe_batch* b, int32_t* done) {
   HIPCHK(hipSetDevice(b->ctx->device));
   const hipError_t e = hipStreamQuery(b->st);
   if (e == hipErrorNotReady) {
+    // not an error: keep it from reading as one at the next launch check
+    (void)hipGetLastError();
     *done = 0;
     return GPE_OK;
   }
