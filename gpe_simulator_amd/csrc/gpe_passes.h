@@ -533,7 +533,14 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   // Only Z passes exchange in rounds: there a wave's lanes own whole
   // consecutive pencils; in the X/Y passes a pencil's threads span waves.
   constexpr int kRounds = AXIS == 2 ? TF::ROUNDS : 1;
-  __shared__ cpx<R> lds[TF::template lds_elems<AXIS>() / kRounds];
+  // Exchange-buffer layout: the pass's own axis, except that the complex64
+  // separable X kinds use the Y passes' rows (unpadded where the block's
+  // pencils span the banks): their exchange pattern is the Y passes', and
+  // the smaller buffer fits more blocks per CU (96^3 batch of 64: xk 218 ->
+  // 182 us).  X_KIN keeps the padded rows (with the smaller buffer it spills
+  // at 96), and so does complex128 (its XK_LAST spills at 96 packed).
+  constexpr int LAX = AXIS == 0 && sizeof(R) == 4 && (KIND == K_XK || KIND == K_XK_LAST) ? 1 : AXIS;
+  __shared__ cpx<R> lds[TF::template lds_elems<LAX>() / kRounds];
   __shared__ typename TF::TwT twl_lds[TF::TWL_ELEMS];
   const SolveDev<R>& s = a.s[blockIdx.z];
   const int n = TF::kFast ? TF::N : s.n;
@@ -562,7 +569,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
   const long long stride = AXIS == 2 ? 1LL : (AXIS == 1 ? (long long)n : (long long)n * n);
   const int ntiles = tile_count<TF, AXIS>(n);
   const int group = b / (TF::B / kRounds);
-  cpx<R>* lp = lds + (b % (TF::B / kRounds)) * TF::template pencil_stride<AXIS>();
+  cpx<R>* lp = lds + (b % (TF::B / kRounds)) * TF::template pencil_stride<LAX>();
   // Block-shared tables staged next to the twiddles (the twiddle barrier
   // covers them): the FP64 x kinetic factors for X_KIN and the z potential
   // line(s) for the Z passes, so they are read from LDS where used instead of
@@ -760,11 +767,11 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
     bool write = true;
 
     if constexpr (KIND == K_Y_FWD || KIND == K_Z_FWD || KIND == K_X_FWD) {
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_INV || KIND == K_Z_INV || KIND == K_X_INV) {
-      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<+1, kRounds, LAX>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_X_KIN) {
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
       if (s.imag) {
         const double eyz = gload(s.kin_y + pj).x * gload(s.kin_z + pk).x;
 #pragma unroll
@@ -797,10 +804,10 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<+1, kRounds, LAX>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID || KIND == K_Z_LAST || KIND == K_Z_HALF) {
       if constexpr (KIND == K_Z_MID || KIND == K_Z_LAST) {
-        TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
+        TF::template fft<+1, kRounds, LAX>(v, lp, t, twp, n, group);
       } else if (prescale) {
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
@@ -901,7 +908,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       } else {
         with_vol(std::false_type{}, std::false_type{});
       }
-      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      if constexpr (KIND == K_Z_FIRST || KIND == K_Z_MID) TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
     } else if constexpr (kKin1) {
       // The separable plan.  Z kinds open with the local half-step(s) in real
       // space, XK_LAST closes with one after its inverse transform; between,
@@ -1025,7 +1032,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
         }
         half_steps();
       }
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
       // The axis's factor.  Imaginary time scales by a real factor (hi + lo:
       // one significant product, so one rounding).  Real time rotates: the
       // product is formed in FP64 from the FP64 table and rounded once
@@ -1051,7 +1058,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           if (m % CHUNK == CHUNK - 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      TF::template fft<+1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<+1, kRounds, LAX>(v, lp, t, twp, n, group);
       if constexpr (kLocalX) half_steps();
     } else if constexpr (KIND == K_Z_DIAG) {
       // Real-space FP64 sums at the diagnostic slot: |psi|^2, V|psi|^2, |psi|^4.
@@ -1074,7 +1081,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
           acc[2] += a2 * a2;
         }
       }
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
     } else if constexpr (KIND == K_Y_LZ || KIND == K_X_LZ) {
       // Angular momentum by Parseval along one axis:
       //   sum_j psi^* (-i d/dy) psi = (1/n) sum_ky k_y |FFT_y psi|^2.
@@ -1082,7 +1089,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
 #pragma unroll
         for (int m = 0; m < M; ++m) v[m] = scale_r(v[m], sc);
       }
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
       const double coord = KIND == K_Y_LZ ? gload(s.xarr + pi) : gload(s.xarr + pj);
       double part = 0.0;
 #pragma unroll
@@ -1095,7 +1102,7 @@ __global__ __launch_bounds__(TF::THREADS, TF::MIN_WAVES) void k_pass(PassArgs<R>
       acc[0] += coord * part;
       write = false;
     } else if constexpr (KIND == K_X_DIAG) {
-      TF::template fft<-1, kRounds, AXIS>(v, lp, t, twp, n, group);
+      TF::template fft<-1, kRounds, LAX>(v, lp, t, twp, n, group);
       const double ky = gload(s.karr + pj), kz = gload(s.karr + pk);
       const int fq = freq_index(pj, n), fr = freq_index(pk, n);
       const float view_scale = s.nview ? float(s.nview) / (1.7320508f * float(s.vh)) : 0.f;
