@@ -23,11 +23,12 @@ REAL, IMAG = 0, 1
 TGT_XY, TGT_Z, TGT_VOL, TGT_VOL_IM = 0, 1, 2, 3
 
 PASS_KINDS = ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin", "z_diag", "x_diag", "z_fwd", "y_lz", "x_lz",
-              "zk_first", "zk_mid", "yk", "xk", "xk_last", "zyk_first", "zyk_mid")
+              "zk_first", "zk_mid", "yk", "xk", "xk_last", "zyk_first", "zyk_mid", "zy_first", "zy_mid", "zy_last")
 # step plans (GPE_PLAN_*): three sweeps per step with the kinetic factor split
-# per axis, the four-sweep 3D-transform plan (the only one with a cutoff), or
-# the separable plan in two sweeps with the z and y axes of each x plane in one
-# pass (sides whose plane fits the LDS)
+# per axis, the four-sweep 3D-transform plan (cutoff allowed), or the plane
+# plan, which runs the z and y axes of each x plane in one pass (sides whose
+# plane fits the LDS): the separable plan in two sweeps, or with a cutoff the
+# four-sweep plan in two (the cutoff is real-time only)
 PLAN_SEPARABLE, PLAN_FOUR_SWEEP, PLAN_PLANE = 0, 1, 2
 
 
@@ -36,14 +37,13 @@ PLANE_SIDES = {0: (32, 48, 64, 96, 128), 1: (32, 48, 64)}
 
 
 def default_plan(precision, kind, masked=False, n=None):
-    """The plan a new solve steps with (gpe_solve_create's rule): four-sweep
-    with a momentum cutoff; the plane plan at the plane sides that are
-    multiples of 64 (faster alone and in batches); else separable."""
-    if masked:
-        return PLAN_FOUR_SWEEP
+    """The plan a new solve steps with (gpe_solve_create's rule): the plane
+    plan at the plane sides that are multiples of 64 (faster alone and in
+    batches; with a momentum cutoff, its ZY passes around the 3D kinetic
+    pass); else four-sweep with a cutoff and separable without."""
     if n is not None and n % 64 == 0 and n in PLANE_SIDES[int(precision)]:
         return PLAN_PLANE
-    return PLAN_SEPARABLE
+    return PLAN_FOUR_SWEEP if masked else PLAN_SEPARABLE
 
 EXPORTS = (
     "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",

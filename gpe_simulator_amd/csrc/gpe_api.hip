@@ -41,7 +41,10 @@ int fail(int code, const std::string& msg) {
     if (r_ != GPE_OK) return r_;   \
   } while (0)
 
-constexpr int kPassKinds = 18;   // GPE_PASS_KINDS: the kinds a step uses (timed and stamped)
+constexpr int kPassKinds = 21;   // GPE_PASS_KINDS: the kinds a step uses (timed and stamped)
+// The plane plan of a solve with the momentum cutoff (internal step plan: the
+// API calls it GPE_PLAN_PLANE).
+constexpr int kPlanPlaneCut = 3;
 // Bytes of time-dependent slot tables per program: a 256^3 complex64 volume
 // program (64 MiB a slot) evaluates 16 slots per 15 steps within it; with 64
 // MiB it evaluated 2 slots every step, half of them again the next step.
@@ -301,8 +304,11 @@ struct gpe_solve {
   int var[kPassKinds] = {0};              // variant used by each pass kind
   // (plane passes are compiled in the first variant's unit of their side)
   const PassVariant& variant_of(int kind) const {
-    return kind == K_ZYK_FIRST || kind == K_ZYK_MID ? vt[0] : vt[var[tuned_kind(kind)]];
+    return is_plane_kind(kind) ? vt[0] : vt[var[tuned_kind(kind)]];
   }
+  // The step plan the passes follow: the plan set, except that a masked
+  // solve on the plane plan runs the ZY kinds (kPlanPlaneCut).
+  int step_plan() const { return plan == GPE_PLAN_PLANE && masked ? kPlanPlaneCut : plan; }
   long long blocks(int kind, int n) const { return variant_of(kind).blocks(kind, n); }
   hipError_t launch(int kind, const void* args, int n, int batch, hipStream_t stream, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) const {
@@ -596,7 +602,7 @@ int pass(gpe_solve* s, int kind, bool diag, int slot_a, int slot_b, int flags, i
     s->stamp_chained = false;
   }
   flags |= s->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
-  const int parts = (flags & F_NORM_FOLD) ? int(s->blocks(norm_kind(s->plan), s->n)) : 0;
+  const int parts = (flags & F_NORM_FOLD) ? int(s->blocks(norm_kind(s->step_plan()), s->n)) : 0;
   PassArgsRaw a{diag ? s->desc_diag : s->desc_step, slot_a, slot_b, flags, hist_index, stamp_index, parts,
                 s->red_override};
   hipEvent_t start = nullptr, stop = nullptr;
@@ -784,9 +790,13 @@ int eval_dynamic(gpe_solve* s, const double* t, int nslots, bool diag, hipStream
 // after it in one plane pass, 2 sweeps per step:
 //   real:      [ZYK_FIRST(0)] XK ZYK_MID(q, q+1)  XK ...  XK_LAST(n-1)
 //   imaginary: ZYK_FIRST(prescale) XK_LAST(norm)     per step
-// Four-sweep plan (GPE_PLAN_FOUR_SWEEP, and masked solves):
+// Four-sweep plan (GPE_PLAN_FOUR_SWEEP, the momentum cutoff's 3D factor):
 //   real:      [Z_FIRST(0)] Y_FWD X_KIN Y_INV Z_MID(q, q+1) ... Z_LAST(n-1)
 //   imaginary: Z_FIRST(prescale) Y_FWD X_KIN Y_INV Z_LAST(norm)
+// Plane plan of a masked solve (kPlanPlaneCut): the four-sweep plan with
+// Y_INV, the Z kind and Y_FWD in one plane pass, 2 sweeps per real step:
+//   real:      [ZY_FIRST(0)] X_KIN ZY_MID(q, q+1)  X_KIN ...  ZY_LAST(n-1)
+// (the cutoff is real-time only, so imaginary steps never take this plan)
 // Real time fuses the closing half-step of step q with the opening one of
 // step q+1; imaginary time ends every step with the FP64 norm and defers the
 // 1/sqrt(norm) scale to the next load (F_PRESCALE).
@@ -795,12 +805,18 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
                bool fold, P&& pass_fn) {
   const bool sep = plan == GPE_PLAN_SEPARABLE;
   const bool plane = plan == GPE_PLAN_PLANE;
+  const bool cut = plan == kPlanPlaneCut;
   for (int64_t q = q0; q < q0 + count; ++q) {
     const int sa = dyn ? (int)(q - q0) : 0, sb = dyn ? sa + 1 : 0;
     const bool first = q == 0, last = q == nsteps - 1;
     if (kind == GPE_REAL) {
       const int pre = first && prescale_first ? F_PRESCALE : 0;
-      if (plane) {
+      if (cut) {
+        if (first) CHK(pass_fn(K_ZY_FIRST, sa, 0, pre, 0));
+        CHK(pass_fn(K_X_KIN, 0, 0, 0, 0));
+        if (last) CHK(pass_fn(K_ZY_LAST, sa, 0, 0, 0));
+        else CHK(pass_fn(K_ZY_MID, sa, sb, 0, 0));
+      } else if (plane) {
         if (first) CHK(pass_fn(K_ZYK_FIRST, sa, 0, pre, 0));
         if (last) {
           CHK(pass_fn(K_XK_LAST, sa, 0, 0, 0));
@@ -853,17 +869,17 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
   return GPE_OK;
 }
 
-// Default step plan: four-sweep with the momentum cutoff, which is not
-// separable; else the plane plan where a plane pass exists and its blocks are
-// whole waves of 8-thread pencils (complex64 at 64 and 128, complex128 at 64:
-// faster alone and in batches); else separable.  At 96 (4 threads a pencil,
+// Default step plan: the plane plan where a plane pass exists and its blocks
+// are whole waves of 8-thread pencils (complex64 at 64 and 128, complex128 at
+// 64: faster alone and in batches), with the ZY kinds under the momentum
+// cutoff, which is not separable; else four-sweep with the cutoff and
+// separable without.  At 96 (4 threads a pencil,
 // 24 values each, one 6-wave block per CU) the plane pass is faster alone but
 // slower in batches, so solves there keep the separable plan unless set.
 // (_lib.default_plan restates this rule.)
 int default_plan(const gpe_solve* s) {
-  if (s->masked) return GPE_PLAN_FOUR_SWEEP;
   if (s->n % 64 == 0 && s->blocks(K_ZYK_MID, s->n) > 0) return GPE_PLAN_PLANE;
-  return GPE_PLAN_SEPARABLE;
+  return s->masked ? GPE_PLAN_FOUR_SWEEP : GPE_PLAN_SEPARABLE;
 }
 
 // The pass kind that writes an imaginary-time step's norm partials.
@@ -876,7 +892,7 @@ int norm_kind(int plan) { return plan == GPE_PLAN_FOUR_SWEEP ? K_Z_LAST : K_XK_L
 // waves.
 constexpr long long kFoldParts = 1024;
 bool norm_fold_ok(const gpe_solve* s) {
-  if (s->blocks(norm_kind(s->plan), s->n) > kFoldParts) return false;
+  if (s->blocks(norm_kind(s->step_plan()), s->n) > kFoldParts) return false;
   if (s->plan == GPE_PLAN_PLANE) return true;   // a plane block is n pencils of TP threads: whole waves
   const PassVariant& v = s->variant_of(s->plan == GPE_PLAN_SEPARABLE ? K_ZK_FIRST : K_Z_FIRST);
   return (v.tp * v.b) % 64 == 0;
@@ -886,13 +902,13 @@ int step_real(gpe_solve* s, int64_t nsteps, const double* tmid) {
   const bool pre = s->pending;
   s->pending = false;
   auto fn = [&](int k, int sa, int sb, int f, int h) { return pass(s, k, false, sa, sb, f, h); };
-  if (!any_dynamic(s)) return emit_steps(GPE_REAL, s->plan, 0, nsteps, nsteps, pre, false, false, fn);
+  if (!any_dynamic(s)) return emit_steps(GPE_REAL, s->step_plan(), 0, nsteps, nsteps, pre, false, false, fn);
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t cs = std::min<int64_t>(s->chunk, nsteps - done);
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     CHK(eval_dynamic(s, tmid + done, nslots, false));
-    CHK(emit_steps(GPE_REAL, s->plan, done, cs, nsteps, pre, true, false, fn));
+    CHK(emit_steps(GPE_REAL, s->step_plan(), done, cs, nsteps, pre, true, false, fn));
     done += cs;
   }
   return GPE_OK;
@@ -950,9 +966,9 @@ int ensure_hist(gpe_solve* s, int64_t nsteps) {
 
 int step_imag(gpe_solve* s, int64_t nsteps) {
   CHK(ensure_hist(s, nsteps));
-  CHK(ensure_red(s, s->blocks(norm_kind(s->plan), s->n)));
+  CHK(ensure_red(s, s->blocks(norm_kind(s->step_plan()), s->n)));
   const bool pre = s->pending;
-  CHK(emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
+  CHK(emit_steps(GPE_IMAG, s->step_plan(), 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   }));
   s->pending = true;
@@ -1190,14 +1206,15 @@ int gpe_solve_set_plan(gpe_solve* s, int32_t plan) {
   if (!s) return fail(GPE_ERR_INVALID, "null solve");
   if (plan != GPE_PLAN_SEPARABLE && plan != GPE_PLAN_FOUR_SWEEP && plan != GPE_PLAN_PLANE)
     return fail(GPE_ERR_INVALID, "unknown step plan");
-  if (plan != GPE_PLAN_FOUR_SWEEP && s->masked)
-    return fail(GPE_ERR_INVALID, "the momentum cutoff is not separable: a masked solve steps with the four-sweep plan");
+  if (plan == GPE_PLAN_SEPARABLE && s->masked)
+    return fail(GPE_ERR_INVALID,
+                "the momentum cutoff is not separable: a masked solve steps with the four-sweep or the plane plan");
   if (plan == GPE_PLAN_PLANE && s->blocks(K_ZYK_MID, s->n) == 0)
     return fail(GPE_ERR_INVALID, "no plane pass at this side and precision (a plane of the field must fit the LDS)");
   CHK(set_device(s));
   HIPCHK(hipStreamSynchronize(s->st));
   s->plan = plan;
-  return ensure_red(s, s->blocks(norm_kind(plan), s->n));
+  return ensure_red(s, s->blocks(norm_kind(s->step_plan()), s->n));
 }
 
 int gpe_solve_get_plan(gpe_solve* s, int32_t* plan) {
@@ -1553,7 +1570,7 @@ int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
   if (!s->ctx->scratch) return fail(GPE_ERR_STATE, "the context holds no scratch address");
   CHK(set_device(s));
   if (s->kind == GPE_REAL) return step_real(s, nsteps, t_mid);   // real time uses no scratch
-  const long long parts = s->blocks(norm_kind(s->plan), s->n);
+  const long long parts = s->blocks(norm_kind(s->step_plan()), s->n);
   if (parts > kScratchDoubles)
     return fail(GPE_ERR_INVALID, "the allocation-free step holds " + std::to_string(kScratchDoubles) +
                                      " norm partials; this solve needs " + std::to_string(parts));
@@ -1562,7 +1579,7 @@ int gpe_step_noalloc(gpe_solve* s, int64_t nsteps, const double* t_mid) {
                                      std::to_string(s->hist_cap) + " steps); the allocation-free step cannot grow it");
   const bool pre = s->pending;
   s->red_override = s->ctx->scratch;
-  const int r = emit_steps(GPE_IMAG, s->plan, 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
+  const int r = emit_steps(GPE_IMAG, s->step_plan(), 0, nsteps, nsteps, pre, false, norm_fold_ok(s), [&](int k, int sa, int sb, int f, int h) {
     return pass(s, k, false, sa, sb, f, h);
   });
   s->red_override = nullptr;
@@ -1821,12 +1838,14 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
     if (!std::equal(s->var, s->var + kPassKinds, b->solves[0]->var))
       return fail(GPE_ERR_INVALID, "batch members must use the same pass variants");
   for (gpe_solve* s : b->solves)
-    if (s->plan != b->solves[0]->plan) return fail(GPE_ERR_INVALID, "batch members must use the same step plan");
+    if (s->step_plan() != b->solves[0]->step_plan())
+      return fail(GPE_ERR_INVALID, "batch members must use the same step plan (and, on the plane plan, all or none "
+                                   "with the momentum cutoff)");
   for (gpe_solve* s : b->solves) {
     s->work_is_spectrum = false;          // the batch advances psi
     if (b->kind == GPE_IMAG) {
       CHK(ensure_hist(s, nsteps));
-      CHK(ensure_red(s, s->blocks(norm_kind(s->plan), s->n)));
+      CHK(ensure_red(s, s->blocks(norm_kind(s->step_plan()), s->n)));
       if (!s->pending) {
         // a unit pending scale lets every member take the prescaled path; the
         // source is a static constant, so it outlives the asynchronous copy
@@ -1851,7 +1870,7 @@ int batch_prepare(gpe_batch* b, int64_t nsteps, bool* prescale_first) {
 
 int batch_pass(gpe_batch* b, int kind, int sa, int sb, int flags, int hist) {
   flags |= b->kind == GPE_IMAG ? F_KIN1_LDS : F_KIN1_RT;
-  const int parts = (flags & F_NORM_FOLD) ? int(b->solves[0]->blocks(norm_kind(b->solves[0]->plan), b->n)) : 0;
+  const int parts = (flags & F_NORM_FOLD) ? int(b->solves[0]->blocks(norm_kind(b->solves[0]->step_plan()), b->n)) : 0;
   PassArgsRaw a{b->desc, sa, sb, flags, hist, 0, parts, nullptr};
   hipEvent_t start = nullptr, stop = nullptr;
   if (b->timing && kind < kPassKinds) HIPCHK(b->timer.begin(kind, b->st, &start, &stop));
@@ -1885,7 +1904,7 @@ int gpe_batch_launch(gpe_batch* b, int64_t nsteps, int32_t use_graph) {
   bool pre = false;
   CHK(batch_prepare(b, nsteps, &pre));
   auto emit = [&]() {
-    return emit_steps(b->kind, b->solves[0]->plan, 0, nsteps, nsteps, pre, false,
+    return emit_steps(b->kind, b->solves[0]->step_plan(), 0, nsteps, nsteps, pre, false,
                       b->kind == GPE_IMAG && norm_fold_ok(b->solves[0]), [&](int k, int sa, int sb, int f, int h) {
       return batch_pass(b, k, sa, sb, f, h);
     });
@@ -1989,7 +2008,7 @@ int gpe_batch_step_times(gpe_batch* b, int64_t nsteps, const double* t_mid, int6
     const int nslots = (int)std::min<int64_t>(cs + 1, nsteps - done);
     for (size_t i = 0; i < b->solves.size(); ++i)
       if (any_dynamic(b->solves[i])) CHK(eval_dynamic(b->solves[i], t_mid + i * stride + done, nslots, false, b->st));
-    CHK(emit_steps(GPE_REAL, b->solves[0]->plan, done, cs, nsteps, pre, true, false, fn));
+    CHK(emit_steps(GPE_REAL, b->solves[0]->step_plan(), done, cs, nsteps, pre, true, false, fn));
     done += cs;
   }
   batch_finish(b, nsteps);

@@ -164,6 +164,18 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
       if (!has_plane<TF>()) return hipErrorInvalidValue;
       go_plane<K_ZYK_MID>(batch, st, a, t0, t1);
       break;
+    case K_ZY_FIRST:
+      if (!has_plane<TF>()) return hipErrorInvalidValue;
+      go_plane<K_ZY_FIRST>(batch, st, a, t0, t1);
+      break;
+    case K_ZY_MID:
+      if (!has_plane<TF>()) return hipErrorInvalidValue;
+      go_plane<K_ZY_MID>(batch, st, a, t0, t1);
+      break;
+    case K_ZY_LAST:
+      if (!has_plane<TF>()) return hipErrorInvalidValue;
+      go_plane<K_ZY_LAST>(batch, st, a, t0, t1);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -172,7 +184,7 @@ hipError_t GPE_LAUNCHER(int kind, const void* args, int n, int batch, hipStream_
 // Blocks per solve that a pass of `kind` launches (sizes partial buffers);
 // 0 for a plane kind that this unit does not compile.
 long long GPE_CAT(pass_blocks_, GPE_SUFFIX)(int kind, int n) {
-  if (kind == K_ZYK_FIRST || kind == K_ZYK_MID) return has_plane<TF>() ? n : 0;
+  if (is_plane_kind(kind)) return has_plane<TF>() ? n : 0;
   const int axis = axis_of(kind);
   return axis == 2 ? tile_count<TF, 2>(n) : (axis == 1 ? tile_count<TF, 1>(n) : tile_count<TF, 0>(n));
 }

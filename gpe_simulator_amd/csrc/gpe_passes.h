@@ -53,27 +53,38 @@ enum PassKind : int {
   // times where a plane fits the LDS.
   K_ZYK_FIRST = 16, // ZK_FIRST then YK
   K_ZYK_MID = 17,   // ZK_MID then YK   (real time)
+  // Plane passes of the momentum-cutoff step (real time): the field between
+  // passes is transformed along y and z, and X_KIN applies the 3D kinetic
+  // factor, so a step sweeps psi twice instead of four times.
+  K_ZY_FIRST = 18,  // Z_FIRST then Y_FWD
+  K_ZY_MID = 19,    // Y_INV then Z_MID then Y_FWD
+  K_ZY_LAST = 20,   // Y_INV then Z_LAST
   // Plain steps of the reference-structured (unfused) real-time step: one
   // launch per reference kernel and transform axis, for measurement.
-  K_Z_HALF = 18,   // half(slot_a) alone, in real space
-  K_Z_INV = 19,    // plain IFFT_z
-  K_X_FWD = 20,    // plain FFT_x
-  K_X_INV = 21,    // plain IFFT_x
+  K_Z_HALF = 21,   // half(slot_a) alone, in real space
+  K_Z_INV = 22,    // plain IFFT_z
+  K_X_FWD = 23,    // plain FFT_x
+  K_X_INV = 24,    // plain IFFT_x
 };
+
+// Kinds run by k_plane (one block per x plane).
+__host__ __device__ constexpr bool is_plane_kind(int kind) {
+  return kind == K_ZYK_FIRST || kind == K_ZYK_MID || kind == K_ZY_FIRST || kind == K_ZY_MID || kind == K_ZY_LAST;
+}
 
 enum PassFlags : int {
   F_PRESCALE = 1,  // multiply the loaded field by scal[1] (pending 1/sqrt(norm))
   F_IN_WORK = 2,   // read from the workspace instead of psi
   F_OUT_WORK = 4,  // write to the workspace instead of psi
-  F_NORM = 8,      // Z_LAST: FP64 norm partials; k_norm_finish computes the scale
+  F_NORM = 8,      // Z_LAST / XK_LAST: FP64 norm partials; k_norm_finish computes the scale
   F_SPECTRUM = 16, // X_DIAG: also bin |F|^2 into the shells and the view
   F_STAMP = 32,    // record the launch's end on the device wall clock
   F_KIN1_LDS = 64, // separable kinds: the launch's dynamic LDS holds the 1D factor table
   F_KIN1_RT = 128, // separable kinds, real time: stage the 1D factor table in LDS where the launcher
                    // finds that it costs no occupancy (the launcher turns it into F_KIN1_LDS)
-  F_NORM_FOLD = 256,  // Z_FIRST / ZK_FIRST / ZYK_FIRST with F_PRESCALE: finish the previous step's norm
+  F_NORM_FOLD = 256,  // the first pass of an imaginary step, with F_PRESCALE: finish the previous step's norm
                       // from its partials (norm_parts of them) in every block, instead of k_norm_finish
-  F_NORM_DEFER = 512, // Z_LAST / XK_LAST with F_NORM: write the partials only; the next step's first
+  F_NORM_DEFER = 512, // the norm kinds with F_NORM: write the partials only; the next step's first
                       // pass finishes the norm (F_NORM_FOLD)
 };
 

@@ -61,13 +61,14 @@ int gpe_device_info(gpe_ctx* ctx, char* name, int32_t name_len, int32_t* runtime
 
 /* Pass tile variants.  A step is a sequence of passes (kinds 0..GPE_PASS_KINDS-1:
  * z_first, z_mid, z_last, y_fwd, y_inv, x_kin, z_diag, x_diag, z_fwd, y_lz,
- * x_lz, zk_first, zk_mid, yk, xk, xk_last, zyk_first, zyk_mid); each pass kind
- * of a solve runs one of the compiled tile variants of its (side, precision),
- * all producing the same transform (the plane kinds zyk_* have one shape per
- * side, whatever the variant).  An autotuner times the variants and sets the
+ * x_lz, zk_first, zk_mid, yk, xk, xk_last, zyk_first, zyk_mid, zy_first,
+ * zy_mid, zy_last); each pass kind of a solve runs one of the compiled tile
+ * variants of its (side, precision), all producing the same transform (the
+ * plane kinds zyk_* and zy_* have one shape per side, whatever the
+ * variant).  An autotuner times the variants and sets the
  * fastest per kind.  No reference counterpart: the reference's FFT plans are
  * cuFFT's (gpe/make_propagator.py:50,52). */
-enum { GPE_PASS_KINDS = 18 };
+enum { GPE_PASS_KINDS = 21 };
 /* Bins of the equal-width spectrum view (gpe_spectrum_view). */
 enum { GPE_VIEW_BINS = 128 };
 /* Number of compiled variants for (side, precision); 1 for sides without a
@@ -115,13 +116,16 @@ int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size
  * local half-steps fused into the z pass (and the last one into the x pass):
  * 3 sweeps of psi per step.  GPE_PLAN_FOUR_SWEEP: the 3D FFT, the kinetic
  * factor and the inverse 3D FFT in four passes (z, y, x, y; 5 in imaginary
- * time).  GPE_PLAN_PLANE: the separable plan with the z and y passes of a
- * step in one, one block per x plane of the field (complex64 at sides 64, 96
- * and 128, complex128 at 64, where a plane fits the LDS): 2 sweeps of psi per
- * step, bit-identical to the separable plan.  A momentum cutoff (kcut_enable)
- * is not separable, so such solves use, and may only use, the four-sweep
- * plan; every other solve starts on the separable plan.  Batch members share
- * a plan. */
+ * time).  GPE_PLAN_PLANE: one block per x plane of the field (complex64 at
+ * sides 32, 48, 64, 96 and 128, complex128 at 32, 48 and 64, where a plane
+ * fits the LDS) runs a step's z and y passes in one: the separable plan's
+ * (2 sweeps of psi per step) or, with a momentum cutoff, the four-sweep
+ * plan's y, z, y passes around the 3D kinetic pass (2 sweeps per step
+ * instead of 4); either matches the plan it replaces to rounding.  A momentum cutoff (kcut_enable) is not separable, so such
+ * solves use the four-sweep or the plane plan.  New solves start on the plane
+ * plan where it exists and the side is a multiple of 64, else on the
+ * four-sweep plan with the cutoff and the separable plan without.  Batch
+ * members share a plan (on the plane plan, all with the cutoff or none). */
 #define GPE_PLAN_SEPARABLE 0
 #define GPE_PLAN_FOUR_SWEEP 1
 #define GPE_PLAN_PLANE 2
