@@ -37,13 +37,15 @@ PLANE_SIDES = {0: (32, 48, 64, 96, 128), 1: (32, 48, 64)}
 
 
 def default_plan(precision, kind, masked=False, n=None):
-    """The plan a new solve steps with (gpe_solve_create's rule): the plane
-    plan at the plane sides that are multiples of 64 (faster alone and in
-    batches; with a momentum cutoff, its ZY passes around the 3D kinetic
-    pass); else four-sweep with a cutoff and separable without."""
-    if n is not None and n % 64 == 0 and n in PLANE_SIDES[int(precision)]:
-        return PLAN_PLANE
-    return PLAN_FOUR_SWEEP if masked else PLAN_SEPARABLE
+    """The plan a new solve steps with (gpe_solve_create's rule).  With a
+    momentum cutoff: the plane plan (its ZY passes around the 3D kinetic
+    pass) at every plane side, else four-sweep.  Without: the plane plan at
+    the plane sides that are multiples of 64 (faster alone and in batches),
+    else separable."""
+    plane = n is not None and n in PLANE_SIDES[int(precision)]
+    if masked:
+        return PLAN_PLANE if plane else PLAN_FOUR_SWEEP
+    return PLAN_PLANE if plane and n % 64 == 0 else PLAN_SEPARABLE
 
 EXPORTS = (
     "gpe_version", "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_mem_info",

@@ -869,17 +869,19 @@ int emit_steps(int kind, int plan, int64_t q0, int64_t count, int64_t nsteps, bo
   return GPE_OK;
 }
 
-// Default step plan: the plane plan where a plane pass exists and its blocks
-// are whole waves of 8-thread pencils (complex64 at 64 and 128, complex128 at
-// 64: faster alone and in batches), with the ZY kinds under the momentum
-// cutoff, which is not separable; else four-sweep with the cutoff and
-// separable without.  At 96 (4 threads a pencil,
+// Default step plan.  With the momentum cutoff, which is not separable: the
+// plane plan (its ZY kinds) wherever a plane pass exists (half the sweeps of
+// the four-sweep plan, faster alone and in batches at every such side), else
+// four-sweep.  Without: the plane plan where a plane pass exists and its
+// blocks are whole waves of 8-thread pencils (complex64 at 64 and 128,
+// complex128 at 64: faster alone and in batches), else separable.  At 96 (4 threads a pencil,
 // 24 values each, one 6-wave block per CU) the plane pass is faster alone but
 // slower in batches, so solves there keep the separable plan unless set.
 // (_lib.default_plan restates this rule.)
 int default_plan(const gpe_solve* s) {
+  if (s->masked) return s->blocks(K_ZY_MID, s->n) > 0 ? GPE_PLAN_PLANE : GPE_PLAN_FOUR_SWEEP;
   if (s->n % 64 == 0 && s->blocks(K_ZYK_MID, s->n) > 0) return GPE_PLAN_PLANE;
-  return s->masked ? GPE_PLAN_FOUR_SWEEP : GPE_PLAN_SEPARABLE;
+  return GPE_PLAN_SEPARABLE;
 }
 
 // The pass kind that writes an imaginary-time step's norm partials.

@@ -122,9 +122,10 @@ int gpe_solve_footprint(const gpe_solve_desc* desc, size_t* resident_bytes, size
  * (2 sweeps of psi per step) or, with a momentum cutoff, the four-sweep
  * plan's y, z, y passes around the 3D kinetic pass (2 sweeps per step
  * instead of 4); either matches the plan it replaces to rounding.  A momentum cutoff (kcut_enable) is not separable, so such
- * solves use the four-sweep or the plane plan.  New solves start on the plane
- * plan where it exists and the side is a multiple of 64, else on the
- * four-sweep plan with the cutoff and the separable plan without.  Batch
+ * solves use the four-sweep or the plane plan.  New solves with the cutoff
+ * start on the plane plan where it exists, else on the four-sweep plan; new
+ * solves without it on the plane plan where it exists and the side is a
+ * multiple of 64, else on the separable plan.  Batch
  * members share a plan (on the plane plan, all with the cutoff or none). */
 #define GPE_PLAN_SEPARABLE 0
 #define GPE_PLAN_FOUR_SWEEP 1

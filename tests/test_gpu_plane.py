@@ -272,6 +272,13 @@ def test_cut_plane_plan_rules(require_gpu):
     batch.close()
     cut.close()
     plain.close()
+    # every plane side takes the plane plan with the cutoff, as the host rule says
+    for n, precision in ((32, "c64"), (48, "c64"), (96, "c64"), (48, "c128"), (128, "c128")):
+        x, k = orc.get_grids(n, 16.0)
+        s = make_solve(n, x, k, "real", 1e-3, 100.0, static=STATIC, kcut=kcut_of(n), precision=precision)
+        prec = _lib.C64 if precision == "c64" else _lib.C128
+        assert s.plan == _lib.default_plan(prec, _lib.REAL, masked=True, n=n), (n, precision)
+        s.close()
     x, k = orc.get_grids(256, 20.0)
     big = make_solve(256, x, k, "real", 1e-3, 100.0, static=STATIC, kcut=kcut_of(256, 20.0), precision="c64")
     assert big.plan == _lib.PLAN_FOUR_SWEEP == _lib.default_plan(_lib.C64, _lib.REAL, masked=True, n=256)

@@ -59,6 +59,25 @@ def test_library_is_gfx950_code():
     assert b"gfx950" in lib.read_bytes()
 
 
+def test_default_plan_rule():
+    """The host's restatement of gpe_solve_create's plan rule (the GPU tests
+    check it against the library): with a cutoff, the plane plan at every
+    side with a plane pass, else four-sweep; without, the plane plan at the
+    plane sides that are multiples of 64, else separable."""
+    from gpe_simulator_amd import _lib
+    for prec, sides in _lib.PLANE_SIDES.items():
+        for n in (16, 32, 48, 64, 96, 128, 192, 256, 512):
+            plane = n in sides
+            assert _lib.default_plan(prec, _lib.REAL, masked=True, n=n) == \
+                (_lib.PLAN_PLANE if plane else _lib.PLAN_FOUR_SWEEP), (prec, n)
+            for kind in (_lib.REAL, _lib.IMAG):
+                assert _lib.default_plan(prec, kind, n=n) == \
+                    (_lib.PLAN_PLANE if plane and n % 64 == 0 else _lib.PLAN_SEPARABLE), (prec, n)
+    assert _lib.default_plan(_lib.C128, _lib.REAL, masked=True, n=128) == _lib.PLAN_FOUR_SWEEP
+    assert _lib.default_plan(_lib.C64, _lib.REAL, masked=True, n=96) == _lib.PLAN_PLANE
+    assert _lib.default_plan(_lib.C64, _lib.REAL, n=96) == _lib.PLAN_SEPARABLE
+
+
 # ------------------------------------------------------------ workflow --
 def write_config(root, stages=1, **first_updates):
     from gpe_simulator_amd.configuration import dump_yaml, new_config
