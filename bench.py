@@ -232,19 +232,22 @@ def stirring(count, n=128, steps=500):
 
 
 def probe_small_grids():
-    """Per shape (64, 96, 128; both precisions and kinds): device time per
-    step of a solo solve between two marks, the sum of its kernels' own
-    durations per step, and host time per step of the same solve as a
-    one-member batch replayed from a hipGraph and launched eagerly."""
+    """Per shape (64, 96, 128; both precisions; imaginary time, real time,
+    and real time with a momentum cutoff at 0.6 k_max): device time per step
+    of a solo solve between two marks, the sum of its kernels' own durations
+    per step, and host time per step of the same solve as a one-member batch
+    replayed from a hipGraph and launched eagerly."""
     from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import make_solve
     from gpe_simulator_amd.grids import get_grids
     rows = []
     for n in (64, 96, 128):
         for prec in ("c64", "c128"):
-            for kind in ("real", "imaginary"):
+            for kind in ("real", "imaginary", "real_cutoff"):
                 x, k = get_grids(n, 20.0)
-                s = make_solve(n, x, k, kind, 5e-4, 300.0, static="x**2 + y**2 + z**2", precision=prec)
+                kcut = 0.6 * np.pi * n / 20.0 if kind == "real_cutoff" else None
+                s = make_solve(n, x, k, kind.split("_")[0], 5e-4, 300.0, static="x**2 + y**2 + z**2", kcut=kcut,
+                               precision=prec)
                 s.gaussian(1.0)
                 s.step(20)
                 steps = 400
