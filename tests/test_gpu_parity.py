@@ -336,9 +336,12 @@ def test_reference_structured_step_is_real_time_only(require_gpu):
     solve.close()
 
 
-def step_counts(plan, kind, steps):
-    """Launches per pass kind of `steps` steps of one call."""
+def step_counts(plan, kind, steps, cut=False):
+    """Launches per pass kind of `steps` steps of one call (`cut`: a solve
+    with the momentum cutoff, real time)."""
     from gpe_simulator_amd import _lib
+    if plan == _lib.PLAN_PLANE and cut:
+        return {"zy_first": 1, "zy_mid": steps - 1, "x_kin": steps, "zy_last": 1}
     if plan == _lib.PLAN_PLANE:
         if kind == "real":
             return {"zyk_first": 1, "zyk_mid": steps - 1, "xk": steps - 1, "xk_last": 1}
@@ -348,19 +351,21 @@ def step_counts(plan, kind, steps):
     return {"zk_first": steps, "yk": steps, "xk_last": steps}
 
 
-@pytest.mark.parametrize("plane", [False, True])
-def test_stamped_step_matches_the_plain_step_and_reports_every_launch(plane, require_gpu):
+@pytest.mark.parametrize("plan_name", ["separable", "plane", "plane_cut"])
+def test_stamped_step_matches_the_plain_step_and_reports_every_launch(plan_name, require_gpu):
     """gpe_step_stamped advances the state exactly as gpe_step and reports
     one positive span per pass launch (time-dependent tables break the chain
     of spans and are not charged to any pass), under the separable and the
-    plane plan."""
+    plane plan, and the plane plan of a solve with the momentum cutoff."""
     from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import midpoint_times
     n, length, dt, steps = 64, 12.0, 1e-3, 7
     beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)"
-    _, stamped = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
-    _, plain = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam)
-    plan = _lib.PLAN_PLANE if plane else _lib.PLAN_SEPARABLE
+    cut = plan_name == "plane_cut"
+    kcut = 0.6 * np.pi * n / length if cut else None
+    _, stamped = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam, kcut)
+    _, plain = make(n, length, "real", dt, 30.0, "x**2 + y**2 + z**2", beam, kcut)
+    plan = _lib.PLAN_SEPARABLE if plan_name == "separable" else _lib.PLAN_PLANE
     psi0 = random_state(n, length, seed=3)
     tm = midpoint_times(0, steps, dt)
     for solve in (stamped, plain):
@@ -370,7 +375,7 @@ def test_stamped_step_matches_the_plain_step_and_reports_every_launch(plane, req
     plain.step(steps, tm)
     np.testing.assert_array_equal(stamped.to_numpy(), plain.to_numpy())
     counts = {k: v[1] for k, v in spans.items() if v[1]}
-    assert counts == step_counts(plan, "real", steps)
+    assert counts == step_counts(plan, "real", steps, cut)
     for kind, (us, count) in spans.items():
         if count:
             assert 0 < us / count < 5e3, (kind, us, count)
@@ -379,20 +384,23 @@ def test_stamped_step_matches_the_plain_step_and_reports_every_launch(plane, req
 
 
 @pytest.mark.parametrize("plane", [False, True])
-@pytest.mark.parametrize("kind", ["real", "imaginary"])
+@pytest.mark.parametrize("kind", ["real", "imaginary", "real_cut"])
 def test_kernel_timed_step_matches_the_plain_step(kind, plane, require_gpu):
     """gpe_step_kernel_times advances the state exactly as gpe_step and
     reports one positive duration per pass launch, each no longer than the
     stream interval around the whole call, under the separable and the plane
-    plan."""
+    plan (real_cut: the momentum cutoff, four-sweep or plane plan)."""
     from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import midpoint_times
     n, length, dt, steps = 64, 12.0, 1e-3, 6
     static = "x**2 + y**2 + z**2"
+    cut = kind == "real_cut"
+    kind = "real" if cut else kind
+    kcut = 0.6 * np.pi * n / length if cut else None
     beam = "10*exp(-((x-2*cos(1.0*t))**2+(y-2*sin(1.0*t))**2)/2)" if kind == "real" else None
-    _, timed = make(n, length, kind, dt, 30.0, static, beam)
-    _, plain = make(n, length, kind, dt, 30.0, static, beam)
-    plan = _lib.PLAN_PLANE if plane else _lib.PLAN_SEPARABLE
+    _, timed = make(n, length, kind, dt, 30.0, static, beam, kcut)
+    _, plain = make(n, length, kind, dt, 30.0, static, beam, kcut)
+    plan = _lib.PLAN_PLANE if plane else (_lib.PLAN_FOUR_SWEEP if cut else _lib.PLAN_SEPARABLE)
     psi0 = random_state(n, length, seed=5)
     tm = midpoint_times(0, steps, dt) if beam else None
     for solve in (timed, plain):
@@ -405,7 +413,11 @@ def test_kernel_timed_step_matches_the_plain_step(kind, plane, require_gpu):
     plain.step(steps, tm)
     np.testing.assert_array_equal(timed.to_numpy(), plain.to_numpy())
     counts = {k: v[1] for k, v in spans.items() if v[1]}
-    assert counts == step_counts(plan, kind, steps)
+    if cut and not plane:
+        assert counts == {"z_first": 1, "y_fwd": steps, "x_kin": steps, "y_inv": steps, "z_mid": steps - 1,
+                          "z_last": 1}
+    else:
+        assert counts == step_counts(plan, kind, steps, cut)
     total = 0.0
     for name, (us, count) in spans.items():
         if count:
