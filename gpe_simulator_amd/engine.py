@@ -34,12 +34,12 @@ def make_solve(n, x_array, k_array, kind, dt, beta, static=None, dynamic=None, k
     kind_code = _lib.IMAG if kind == "imaginary" else _lib.REAL
     ctx = ctx or _lib.default_context()
     # pass variants: tuned once per context, shape and step plan, then the
-    # context default.  With a cutoff the one pass kind whose tile varies is
-    # x_kin, on either plan (the plane passes have one shape), and it is tuned
-    # within the four-sweep step.
+    # context default (with a cutoff on the plane plan, x_kin alone: the
+    # plane passes have one shape per side)
     plan_code = _lib.default_plan(precision_code(precision), kind_code, masked=kcut is not None, n=n)
-    tune.ensure_tuned(n, precision_code(precision), kind_code, ctx,
-                      plan=_lib.PLAN_FOUR_SWEEP if kcut is not None else plan_code)
+    if kcut is not None and plan_code == _lib.PLAN_PLANE:
+        plan_code = tune.PLAN_PLANE_CUT
+    tune.ensure_tuned(n, precision_code(precision), kind_code, ctx, plan=plan_code)
     solve = _lib.Solve(n, precision_code(precision), kind_code, dt, beta, x_array, k_array, kcut=kcut, ctx=ctx)
     plan = plan_potential(static, dynamic if kind_code == _lib.REAL else None)
     for target, program in plan.static.items():

@@ -1,7 +1,9 @@
 """Autotuner for the pass tile variants.
 
 Every pass kind of a step (separable plan: zk_first, zk_mid, yk, xk, xk_last;
-four-sweep plan: z_first, z_mid, z_last, y_fwd, y_inv, x_kin) reads and writes
+four-sweep plan: z_first, z_mid, z_last, y_fwd, y_inv, x_kin; the plane
+plan's plane passes have one shape per side, so it tunes xk and xk_last, or
+with a momentum cutoff x_kin alone) reads and writes
 the field in the same canonical layout, so each kind can run any of the
 compiled tile variants of its (side, precision): threads per pencil, pencils
 per block and the waves per SIMD its register budget allows (gpe_shapes.h).
@@ -37,10 +39,15 @@ from dataclasses import dataclass, field
 from . import _lib
 from .grids import get_grids
 
+# The plane plan of a solve with a momentum cutoff (a tuner key only; the
+# library calls it PLAN_PLANE): its ZY plane passes have one shape per side,
+# so x_kin is the one kind to tune, in scratch solves that carry a cutoff.
+PLAN_PLANE_CUT = "plane_cut"
 STEP_KINDS = {_lib.PLAN_SEPARABLE: ("zk_first", "zk_mid", "yk", "xk", "xk_last"),
               _lib.PLAN_FOUR_SWEEP: ("z_first", "z_mid", "z_last", "y_fwd", "y_inv", "x_kin"),
               # (the plane passes have one shape per side; only the x passes vary)
-              _lib.PLAN_PLANE: ("xk", "xk_last")}
+              _lib.PLAN_PLANE: ("xk", "xk_last"),
+              PLAN_PLANE_CUT: ("x_kin",)}
 # pass kinds of a step in the order the descent visits them (most time first)
 STEP_SEQUENCE = {
     _lib.PLAN_SEPARABLE: {_lib.REAL: ("zk_mid", "xk", "yk", "zk_first", "xk_last"),
@@ -48,8 +55,10 @@ STEP_SEQUENCE = {
     _lib.PLAN_FOUR_SWEEP: {_lib.REAL: ("z_mid", "x_kin", "y_fwd", "y_inv", "z_first", "z_last"),
                            _lib.IMAG: ("z_first", "z_last", "x_kin", "y_fwd", "y_inv")},
     _lib.PLAN_PLANE: {_lib.REAL: ("xk", "xk_last"), _lib.IMAG: ("xk_last",)},
+    PLAN_PLANE_CUT: {_lib.REAL: ("x_kin",)},
 }
-PLAN_NAMES = {_lib.PLAN_SEPARABLE: "separable", _lib.PLAN_FOUR_SWEEP: "four-sweep", _lib.PLAN_PLANE: "plane"}
+PLAN_NAMES = {_lib.PLAN_SEPARABLE: "separable", _lib.PLAN_FOUR_SWEEP: "four-sweep", _lib.PLAN_PLANE: "plane",
+              PLAN_PLANE_CUT: "plane with cutoff"}
 GAIN = 0.005        # a descent move must shorten the step by more than this fraction
 _lock = threading.Lock()
 
@@ -86,11 +95,17 @@ def enabled():
     return os.environ.get("GPE_HIP_AUTOTUNE", "1") != "0"
 
 
+def _plan_code(plan):
+    """A tuner plan key as the library's plan (or the tuner-only key)."""
+    return plan if plan == PLAN_PLANE_CUT else int(plan)
+
+
 def _scratch(n, precision, kind, ctx, plan):
     length = 20.0
     x, k = get_grids(n, length)
-    s = _lib.Solve(n, precision, kind, 5e-4, 100.0, x, k, ctx=ctx)
-    s.set_plan(plan)
+    cut = plan == PLAN_PLANE_CUT
+    s = _lib.Solve(n, precision, kind, 5e-4, 100.0, x, k, kcut=0.6 * k.max() if cut else None, ctx=ctx)
+    s.set_plan(_lib.PLAN_PLANE if cut else plan)
     from .expressions import plan_potential
     for target, program in plan_potential("x**2 + y**2 + z**2", None).static.items():
         s.potential_static(target, program)
@@ -209,7 +224,7 @@ def autotune(n, precision, kind, steps=12, batch=1, rounds=3, ctx=None, plan=Non
     Tuning).  Kinds outside the plan keep `start` (default: variant 0)."""
     precision = int(precision)
     kind = int(kind)
-    plan = _lib.default_plan(precision, kind, n=n) if plan is None else int(plan)
+    plan = _lib.default_plan(precision, kind, n=n) if plan is None else _plan_code(plan)
     ctx = ctx or _lib.default_context()
     variants, timings = measure(n, precision, kind, steps=steps, batch=batch, rounds=rounds, ctx=ctx, plan=plan)
     event_choice = list(start) if start is not None else [0] * len(_lib.PASS_KINDS)
@@ -237,7 +252,7 @@ def ensure_tuned(n, precision, kind, ctx=None, batch=1, plan=None):
         return None
     ctx = ctx or _lib.default_context()
     n, precision, kind = int(n), int(precision), int(kind)
-    plan = _lib.default_plan(precision, kind, n=n) if plan is None else int(plan)
+    plan = _lib.default_plan(precision, kind, n=n) if plan is None else _plan_code(plan)
     key = (n, precision, kind, int(batch)) if plan == _lib.default_plan(precision, kind, n=n) else \
         (n, precision, kind, int(batch), plan)
     with _lock:

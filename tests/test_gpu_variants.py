@@ -112,6 +112,28 @@ def test_autotune_sets_context_defaults(require_gpu):
     ctx.close()
 
 
+def test_cutoff_plane_solves_tune_x_kin_alone(require_gpu):
+    """A solve with the momentum cutoff on the plane plan tunes x_kin, in
+    scratch solves with a cutoff on the plane plan, and no other kind: the
+    plane passes have one shape per side."""
+    from gpe_simulator_amd import _lib, tune
+    from gpe_simulator_amd.engine import make_solve
+    ctx = _lib.Context(0)
+    x, k = orc.get_grids(64, 16.0)
+    s = make_solve(64, x, k, "real", 1e-3, 100.0, static="x**2 + y**2 + z**2", kcut=0.6 * np.abs(k).max(),
+                   precision="c64", ctx=ctx)
+    assert s.plan == _lib.PLAN_PLANE
+    result = ctx.tunings[(64, _lib.C64, _lib.REAL, 1, tune.PLAN_PLANE_CUT)]
+    print(result.report(), flush=True)
+    assert result.plan == tune.PLAN_PLANE_CUT
+    x_kin = _lib.PASS_KINDS.index("x_kin")
+    assert all(v == 0 for i, v in enumerate(result.choice) if i != x_kin)
+    assert set(result.timings[0]) == {"x_kin"}
+    assert s.variants()[x_kin] == result.choice[x_kin]
+    s.close()
+    ctx.close()
+
+
 def test_batch_members_must_share_variants(require_gpu):
     from gpe_simulator_amd import _lib
     from gpe_simulator_amd.engine import make_solve
