@@ -79,6 +79,32 @@ def test_pass_kernels_spill_only_at_their_register_cap(tmp_path):
     assert not structural, "\n".join(structural)
 
 
+@pytest.mark.skipif(not all(p.exists() for p in TOOLS.values()), reason="ROCm LLVM tools not installed")
+def test_plane_kernels_spill_only_at_their_register_cap(tmp_path):
+    """The plane passes (one block of n pencils x TP threads per x plane,
+    compiled in each side's first variant) likewise: a block of T threads
+    puts ceil(T / 256) waves on each SIMD, so the cap is 512 over that."""
+    variants = current_variants()
+    objects = [BUILD / f"gpe_pass_p{p}_n{n}_v0.o" for (p, n, v) in variants if v == 0 and n]
+    if not all(o.exists() for o in objects):
+        pytest.skip("library not built (python -m gpe_simulator_amd.build)")
+    checked, structural = 0, []
+    for obj in objects:
+        for name, scratch, vgpr in kernel_metadata(obj, tmp_path):
+            m = re.search(r"k_plane<(?:float|double), (\d+), (\d+), (\d+)>", name) or \
+                re.search(r"k_planeI[fd]Li(\d+)ELi(\d+)ELi(\d+)E", name)
+            if not m:
+                continue
+            n, tp = int(m.group(1)), int(m.group(2))
+            waves_per_simd = -(-(n * tp) // 256)
+            cap = min(512, (512 // waves_per_simd) // 8 * 8)
+            checked += 1
+            if scratch and (vgpr is None or vgpr < cap - 8):
+                structural.append(f"{obj.name}: {name} scratch {scratch} B/lane at {vgpr} of {cap} VGPRs")
+    assert checked >= 5 * 5      # five plane kinds at each of at least five (precision, side)
+    assert not structural, "\n".join(structural)
+
+
 def test_headline_kernels_use_no_scratch(tmp_path):
     """The 256^3 variants (both precisions) run without any scratch."""
     variants = current_variants()
