@@ -3,7 +3,8 @@
 // The separable plan (gpe/make_propagator.py:38-101 restated as one FFT ->
 // 1D factor -> inverse FFT pass per axis) reads and writes psi once per axis.
 // Where one x plane of the field (n^2 values) fits a CU's LDS (complex64 at
-// sides 64, 96 and 128, complex128 at 64), one block takes the whole plane
+// sides 32, 48, 64, 96 and 128, complex128 at 32, 48 and 64), one block
+// takes the whole plane
 // and runs ZK_FIRST / ZK_MID and then YK on it: the z pencils are loaded,
 // given their local half-step(s), transformed along z, scaled by the kz
 // factor and transformed back; the plane then turns through LDS so that the
