@@ -145,8 +145,15 @@ __global__ __launch_bounds__(TP * N) void k_plane(PassArgs<R> a) {
 #pragma unroll
     for (int m = 0; m < M; ++m) lds[yz * (N + TP) + TF::pos(tz, m)] = v[m];
     __syncthreads();
+    // ZY_MID wrote these same elements in its first turn (y_to_z).  Their
+    // addresses beyond the 64 KiB reach of a ds offset take one VGPR each, and
+    // the compiler kept them from the first turn to here (at 128 complex64,
+    // ten of them, spilled: 5 MiB of scratch writes per launch).  The empty
+    // asm hides that this column is the same value, so they are formed again.
+    int col = zy;
+    if constexpr (KIND == K_ZY_MID) asm volatile("" : "+v"(col));
 #pragma unroll
-    for (int m = 0; m < M; ++m) v[m] = lds[TF::pos(ty, m) * (N + TP) + zy];
+    for (int m = 0; m < M; ++m) v[m] = lds[TF::pos(ty, m) * (N + TP) + col];
   };
   auto y_to_z = [&](cpx<R>* v) {
     __syncthreads();
